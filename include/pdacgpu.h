@@ -1,0 +1,160 @@
+/*
+ * pdacgpu.h -- C-ABI of the MI355X (gfx950) implementation of PDAC's
+ * per-time-step hot path.
+ *
+ * The reference solver exposes its hot path as argument-less Fortran module
+ * subroutines that read and write module-global arrays (tilde.f:125,276;
+ * iter.f:38; ygas.f:27; htilde.f:25; ftem.f:19; bdry.f:22), called from the
+ * time loop of main_prog (prog.f:143-417).  This header is the drop-in
+ * boundary for that loop: one context owns every field in device memory, the
+ * host (Fortran through iso_c_binding, C, or Python through ctypes) uploads
+ * the state once, calls the same subroutines in the same order, and downloads
+ * fields when it writes output or restart files.
+ *
+ * Conventions
+ *   - every function returns int: 0 ok, >0 numerical status (see PDAC_ST_*),
+ *     <0 HIP/runtime error (see pdac_last_error());
+ *   - host arrays are Fortran column-major (ntot, ncomp): component c of
+ *     cell ijk (0-based, i fastest) is at [c*ntot + ijk];
+ *   - cell index ijk = (i-1) + (j-1)*nx + (k-1)*nx*ny with 1-based i,j,k as in
+ *     the reference (ghost.f:817-864), nx,ny,nz including boundary cells;
+ *   - one host thread per context; calls are stream-ordered on the context's
+ *     HIP stream and only the calls that return scalars synchronise.
+ */
+#ifndef PDACGPU_H
+#define PDACGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDAC_MAX_SOLID 8
+#define PDAC_MAX_GAS 7
+
+/* numerical status codes (>0) */
+#define PDAC_ST_ITER_NOT_CONVERGED 1  /* iter.f:428-450 with cstop */
+#define PDAC_ST_EOS_FAILURE 2         /* ftem.f:191-193 */
+#define PDAC_ST_BAD_ARGUMENT 3
+
+/* precision forms */
+#define PDAC_PREC_FP64 0      /* plain FP64 drop-in */
+#define PDAC_PREC_FP32FLUX 1  /* MUSCL/limiter/flux arithmetic in FP32 */
+
+/* iteration orders of the pressure-velocity sweep */
+#define PDAC_ORDER_LEXICOGRAPHIC 0 /* reference Gauss-Seidel order (CPU only) */
+#define PDAC_ORDER_REDBLACK 1      /* two-colour order used on the GPU */
+
+typedef struct pdac_dims {
+  int32_t job_type;  /* 2 or 3 (control_flags JOB_TYPE_2D/3D) */
+  int32_t itc;       /* 1: 2D cylindrical (grid.f:319-340) */
+  int32_t nx, ny, nz;/* cells per axis including boundary cells; ny=1 in 2D */
+  int32_t nsolid;    /* particle classes (<= PDAC_MAX_SOLID) */
+  int32_t ngas;      /* gas species (<= PDAC_MAX_GAS) */
+  int32_t precision; /* PDAC_PREC_* */
+} pdac_dims;
+
+/* Run-time constants: the &model/&numeric/&particles/&atmosphere namelist
+ * values the hot path reads (input.f:159-209). */
+typedef struct pdac_params {
+  double alpha, alphagrav;          /* implicitness of pressure / gravity */
+  double gravx, gravy, gravz;
+  double beta;                      /* ultrabeta / beta limiter parameter */
+  double omega, delg;               /* SOR relaxation, residual tolerance */
+  double rlim, flim;                /* matrix limits (matrix.f, ftem.f) */
+  double tref;                      /* reference temperature for icpc<=1 */
+  double void_fraction;             /* atmospheric void fraction (inflow) */
+  double dk[PDAC_MAX_SOLID];        /* particle diameter [m] */
+  double rl[PDAC_MAX_SOLID];        /* particle density */
+  double phis[PDAC_MAX_SOLID];      /* sphericity */
+  double cps[PDAC_MAX_SOLID];       /* particle specific heat */
+  double kap[PDAC_MAX_SOLID];       /* particle conductivity */
+  double plim[PDAC_MAX_SOLID];      /* two-phase limit */
+  double atm_ygc[PDAC_MAX_GAS];     /* atmospheric mass fractions by type */
+  int32_t gas_type[PDAC_MAX_GAS];   /* 1-based reference gas types */
+  int32_t muscl, lm, lv;            /* MUSCL on/off, mass / velocity limiter */
+  int32_t inmax, maxout;            /* in-cell and outer iteration limits */
+  int32_t pmodel, drag_model, icpc, ord;
+  int32_t cstop;                    /* stop when iter does not converge */
+  int32_t iter_order;               /* PDAC_ORDER_* */
+  int32_t pad_;
+} pdac_params;
+
+/* Field identifiers for pdac_put/pdac_get.  ncomp is 1 unless noted
+ * (S = nsolid, G = ngas, A = (S+1)(S+2)/2 packed interphase entries). */
+enum pdac_field {
+  PDAC_P = 0, PDAC_EP, PDAC_RGP, PDAC_ROG, PDAC_TG, PDAC_SIEG,
+  PDAC_UG, PDAC_VG, PDAC_WG,
+  PDAC_RLK,   /* S */
+  PDAC_TS,    /* S */
+  PDAC_SIES,  /* S */
+  PDAC_US, PDAC_VS, PDAC_WS, /* S each */
+  PDAC_YGC, PDAC_XGC,        /* G each */
+  PDAC_CG, PDAC_CK /* S */, PDAC_MUG, PDAC_KAPG,
+  /* step-start (n-level) copies written by fieldn */
+  PDAC_PN, PDAC_EPN, PDAC_RGPN, PDAC_SIEGN, PDAC_RLKN /* S */, PDAC_SIESN /* S */,
+  PDAC_RUGN, PDAC_RVGN, PDAC_RWGN, PDAC_RUSN, PDAC_RVSN, PDAC_RWSN /* S */,
+  PDAC_RGPGCN /* G */,
+  /* explicit (tilde) terms and interphase matrices written by tilde/htilde */
+  PDAC_RUG, PDAC_RVG, PDAC_RWG, PDAC_RUS, PDAC_RVS, PDAC_RWS /* S */,
+  PDAC_APPU, PDAC_APPV, PDAC_APPW /* A */,
+  PDAC_RHG, PDAC_RHS /* S */,
+  /* mass fluxes of the last iter call */
+  PDAC_RGFE, PDAC_RGFN, PDAC_RGFT, PDAC_RSFE, PDAC_RSFN, PDAC_RSFT /* S */,
+  PDAC_NFIELDS
+};
+
+typedef struct pdac_ctx pdac_ctx;
+
+/* Context lifetime.  device < 0 selects the current device. */
+int pdac_create(pdac_ctx **ctx, const pdac_dims *dims, int device);
+int pdac_destroy(pdac_ctx *ctx);
+
+/* Grid: cell sizes dx[nx], dy[ny] (ignored in 2D), dz[nz]; cell flags[ntot]
+ * (grid.f:45-58 values); atmospheric profile p_atm[nz], t_atm[nz] used by
+ * free in/outflow boundaries (inoutflow.f:60-123).  r, rb, inr, inrb and the
+ * inverse sizes are derived as in grid_setup (grid.f:284-340). */
+int pdac_set_grid(pdac_ctx *ctx, const double *dx, const double *dy,
+                  const double *dz, const int32_t *flag,
+                  const double *p_atm, const double *t_atm);
+int pdac_set_params(pdac_ctx *ctx, const pdac_params *prm);
+
+/* Host <-> device copies of one field; n must equal ntot*ncomp(field). */
+int pdac_put(pdac_ctx *ctx, int field, const double *host, int64_t n);
+int pdac_get(pdac_ctx *ctx, int field, double *host, int64_t n);
+int pdac_field_ncomp(const pdac_ctx *ctx, int field);
+
+/* Derived state after a restart read: molar fractions, void fraction and
+ * heat capacities (cnvert itd=2, setup.f:359-392). */
+int pdac_cnvert(pdac_ctx *ctx);
+
+/* The hot path, one reference subroutine each. */
+int pdac_boundary(pdac_ctx *ctx);                       /* bdry.f:22 */
+int pdac_fieldn(pdac_ctx *ctx);                         /* tilde.f:125 */
+int pdac_tilde(pdac_ctx *ctx, double dt);               /* tilde.f:276 */
+int pdac_iter(pdac_ctx *ctx, double dt, int32_t *nit,
+              int32_t *converged);                      /* iter.f:38 */
+int pdac_ygas(pdac_ctx *ctx, double dt);                /* ygas.f:27 */
+int pdac_htilde(pdac_ctx *ctx, double dt);              /* htilde.f:25 */
+int pdac_ftem(pdac_ctx *ctx, double dt, int32_t *eos_fail); /* ftem.f:19 */
+
+/* Reset of the per-step tilde/interphase arrays (allocate_fluxes,
+ * tilde.f:56-103); pdac_step calls it once per step. */
+int pdac_begin_step(pdac_ctx *ctx);
+
+/* nsteps full semi-implicit steps of main_prog (prog.f:143-329):
+ * boundary, fieldn, then for rk = 1..rungekut with dt/(rungekut+1-rk):
+ * tilde, iter, ygas, htilde, ftem.  nit_last receives the sweep count of the
+ * last iter call, nit_total the sum over all calls. */
+int pdac_step(pdac_ctx *ctx, double dt, int32_t rungekut, int32_t nsteps,
+              int32_t *nit_last, int64_t *nit_total);
+
+int pdac_sync(pdac_ctx *ctx);
+const char *pdac_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDACGPU_H */

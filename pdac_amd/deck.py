@@ -1,0 +1,230 @@
+"""Case description and ``pdac.dat`` input-deck writer.
+
+A :class:`Case` holds what the reference reads from its input deck
+(namelists ``&control &model &mesh &boundaries &topography &inlet &dome
+&atmosphere &particles &numeric`` and the ``MESH`` card, input.f:159-209 and
+input.f:790-823) plus the initial primary fields, which enter through a
+``pdac.res`` restart file (``restart_mode='restart'``).  The same object drives
+the GPU solver, so a deck run by the reference and a run on the GPU start from
+identical bits.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import constants as C
+from .restart import RestartState, write_restart
+
+
+@dataclass
+class Case:
+    job_type: str = "2D"          # '2D' or '3D'
+    itc: int = 0                  # 1: 2D cylindrical
+    nx: int = 10
+    ny: int = 1
+    nz: int = 10
+    dx: Optional[np.ndarray] = None   # cell sizes including boundary cells
+    dy: Optional[np.ndarray] = None
+    dz: Optional[np.ndarray] = None
+    # boundary types as written in the deck (flag = 2**(type-1)), grid.f:385-470
+    west: int = 2
+    east: int = 2
+    south: int = 2
+    north: int = 2
+    bottom: int = 2
+    top: int = 2
+    ord: int = 0
+    bl_row0: int = 1
+    # particles (input.f:399-407 names)
+    nsolid: int = 1
+    diameter: List[float] = field(default_factory=lambda: [100.0])
+    density: List[float] = field(default_factory=lambda: [2500.0])
+    sphericity: List[float] = field(default_factory=lambda: [1.0])
+    viscosity: List[float] = field(default_factory=lambda: [0.5])
+    specific_heat: List[float] = field(default_factory=lambda: [1200.0])
+    thermal_conductivity: List[float] = field(default_factory=lambda: [2.0])
+    twophase_limit: List[float] = field(default_factory=lambda: [1.0e-8])
+    # gas: species present, as reference gas-type indices (1-based)
+    gas_type: List[int] = field(default_factory=lambda: [5, 6])
+    # model
+    icpc: int = 2
+    tref: float = 298.0
+    drag_model: int = 1
+    pmodel: int = 2
+    gravx: float = 0.0
+    gravy: float = 0.0
+    gravz: float = 0.0
+    # atmosphere used by free in/outflow boundaries
+    p_ground: float = 1.01325e5
+    t_ground: float = 288.15
+    stratification: bool = False
+    void_fraction: float = 1.0 - 1.0e-10
+    atm_ygc: List[float] = field(default_factory=lambda: [0, 0, 0, 0, 0, 1.0, 0])
+    # numerics (input.f:452-466)
+    time_integration: int = 0
+    rungekut: int = 1
+    beta: float = 0.25
+    muscl: int = 1
+    mass_limiter: int = 2
+    vel_limiter: int = 2
+    inmax: int = 8
+    maxout: int = 500
+    delg: float = 1.0e-8
+    omega: float = 1.1
+    rlim: float = 1.0e-8
+    flim: float = 1.0e-6
+    alpha: float = 1.0
+    alphagrav: float = 0.0
+    # control
+    dt: float = 1.0e-5
+    nsteps: int = 1
+    time: float = 0.0
+    cstop: bool = False
+    # initial primary fields, keyed like RestartState.fields
+    fields: Dict[str, np.ndarray] = field(default_factory=dict)
+
+    # ------------------------------------------------------------------
+    @property
+    def three_d(self) -> bool:
+        return self.job_type.upper() == "3D"
+
+    @property
+    def ntot(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def cell_sizes(self):
+        dx = np.asarray(self.dx, dtype=np.float64)
+        dz = np.asarray(self.dz, dtype=np.float64)
+        dy = np.asarray(self.dy, dtype=np.float64) if self.three_d else np.zeros(1)
+        return dx, dy, dz
+
+    def flags(self) -> np.ndarray:
+        """Cell flags, i fastest: ``flic`` (grid.f:375-470) followed by the
+        near-ground layer of ``export_topography`` (topo.f:1116-1162).
+
+        The latter turns the fluid cells of each column that lie within three
+        rows above the last no-slip cell met so far into ``bl_cell`` (5),
+        which disables the MUSCL corrections there.  Its row counter is not
+        initialised in the reference and carries over from column to column;
+        it starts at ``bl_row0`` here (1, the bottom boundary row, which is
+        what the reference build does for a deck whose bottom is not a no-slip
+        wall)."""
+        nx, ny, nz = self.nx, self.ny, self.nz
+        if not self.three_d:
+            fl = np.full((nz, 1, nx), C.FLUID, dtype=np.int32)
+            fl[0, 0, 1:nx - 1] = 2 ** (self.bottom - 1)
+            fl[nz - 1, 0, 1:nx - 1] = 2 ** (self.top - 1)
+            fl[:, 0, 0] = 2 ** (self.west - 1)
+            fl[:, 0, nx - 1] = 2 ** (self.east - 1)
+        else:
+            fl = np.full((nz, ny, nx), C.FLUID, dtype=np.int32)
+            fl[0, 1:ny - 1, 1:nx - 1] = 2 ** (self.bottom - 1)
+            fl[nz - 1, 1:ny - 1, 1:nx - 1] = 2 ** (self.top - 1)
+            fl[:, :, 0] = 2 ** (self.west - 1)
+            fl[:, :, nx - 1] = 2 ** (self.east - 1)
+            fl[:, 0, :] = 2 ** (self.south - 1)
+            fl[:, ny - 1, :] = 2 ** (self.north - 1)
+        q = self.bl_row0
+        for j in range(fl.shape[1]):
+            for i in range(nx):
+                col = fl[:, j, i]
+                for k in range(nz):
+                    if col[k] == C.NOSLIP_WALL:
+                        q = k + 1
+                    elif col[k] == C.FLUID and (k + 1) <= q + 3:
+                        col[k] = C.BL_CELL
+        return fl.reshape(-1)
+
+    # ------------------------------------------------------------------
+    def restart_state(self) -> RestartState:
+        st = RestartState(self.time, self.nx, self.ny, self.nz, self.nsolid,
+                          len(self.gas_type), 0, list(self.gas_type),
+                          self.three_d)
+        st.fields = {k: np.array(v, dtype=np.float64) for k, v in self.fields.items()}
+        return st
+
+    def write_restart(self, path: str) -> None:
+        write_restart(path, self.restart_state())
+
+    def write_deck(self, path: str) -> None:
+        def b(x):
+            return "T" if x else "F"
+
+        def d(x):
+            return repr(float(x)).replace("e", "D") if "e" in repr(float(x)) else repr(float(x)) + "D0"
+
+        def arr(xs):
+            return ", ".join(d(x) for x in xs)
+
+        tstop = self.time + self.nsteps * self.dt
+        lines = []
+        lines.append("&control")
+        lines.append(f" run_name = 'graft', job_type = '{self.job_type}',")
+        lines.append(" restart_mode = 'restart',")
+        lines.append(f" time = {d(self.time)}, tstop = {d(tstop)}, dt = {d(self.dt)},")
+        lines.append(f" lpr = 0, imr = 0, tpr = {d(1.0e6)}, tdump = {d(self.nsteps * self.dt)},")
+        lines.append(f" nfil = 0, max_seconds = {d(1.0e7)}, cstop = {b(self.cstop)}")
+        lines.append("/")
+        lines.append("&model")
+        lines.append(f" icpc = {self.icpc}, tref = {d(self.tref)}, irex = 1,")
+        lines.append(" gas_viscosity = F, part_viscosity = F, iss = 0, iturb = 0,")
+        lines.append(f" drag_model = {self.drag_model}, pmodel = {self.pmodel},")
+        lines.append(f" gravx = {d(self.gravx)}, gravy = {d(self.gravy)}, gravz = {d(self.gravz)},")
+        lines.append(f" ngas = {len(self.gas_type)}, isink = 0")
+        lines.append("/")
+        lines.append("&mesh")
+        dx, dy, dz = self.cell_sizes()
+        lines.append(f" nx = {self.nx}, ny = {self.ny}, nz = {self.nz}, itc = {self.itc},")
+        lines.append(" iuni = 0, grigen = 0, zzero = 0.0D0")
+        lines.append("/")
+        lines.append("&boundaries")
+        lines.append(f" west = {self.west}, east = {self.east}, south = {self.south},")
+        lines.append(f" north = {self.north}, bottom = {self.bottom}, top = {self.top},")
+        lines.append(f" immb = 0, ibl = 0, ord = {self.ord}")
+        lines.append("/")
+        lines.append("&topography")
+        lines.append(" itp = 0")
+        lines.append("/")
+        lines.append("&inlet")
+        lines.append(" ivent = 0")
+        lines.append("/")
+        lines.append("&dome")
+        lines.append(" idome = 0")
+        lines.append("/")
+        lines.append("&atmosphere")
+        lines.append(f" p_ground = {d(self.p_ground)}, t_ground = {d(self.t_ground)},")
+        lines.append(f" stratification = {b(self.stratification)}, void_fraction = {d(self.void_fraction)},")
+        names = ["atm_O2", "atm_N2", "atm_CO2", "atm_H2", "atm_H2O", "atm_Air", "atm_SO2"]
+        lines.append(" " + ", ".join(f"{n} = {d(v)}" for n, v in zip(names, self.atm_ygc)))
+        lines.append("/")
+        lines.append("&particles")
+        lines.append(f" nsolid = {self.nsolid},")
+        lines.append(f" diameter = {arr(self.diameter)},")
+        lines.append(f" density = {arr(self.density)},")
+        lines.append(f" sphericity = {arr(self.sphericity)},")
+        lines.append(f" viscosity = {arr(self.viscosity)},")
+        lines.append(f" specific_heat = {arr(self.specific_heat)},")
+        lines.append(f" thermal_conductivity = {arr(self.thermal_conductivity)},")
+        lines.append(f" twophase_limit = {arr(self.twophase_limit)}")
+        lines.append("/")
+        lines.append("&numeric")
+        lines.append(f" time_integration = {self.time_integration}, rungekut = {self.rungekut},")
+        lines.append(f" beta = {d(self.beta)}, muscl = {self.muscl}, mass_limiter = {self.mass_limiter},")
+        lines.append(f" vel_limiter = {self.vel_limiter}, inmax = {self.inmax}, maxout = {self.maxout},")
+        lines.append(f" omega = {d(self.omega)}, delg = {d(self.delg)}, optimization = 1,")
+        lines.append(" implicit_fluxes = F, implicit_enthalpy = F, update_eosg = F,")
+        lines.append(f" rlim = {d(self.rlim)}, flim = {d(self.flim)}, alpha = {d(self.alpha)},")
+        lines.append(f" alphagrav = {d(self.alphagrav)}, ctu = 0, adapt_dt = F")
+        lines.append("/")
+        lines.append("'MESH'")
+        lines.append(" " + " ".join(d(x) for x in dx))
+        if self.three_d:
+            lines.append(" " + " ".join(d(x) for x in dy))
+        lines.append(" " + " ".join(d(x) for x in dz))
+        lines.append("'FIXED_FLOWS'")
+        lines.append(" 0")
+        with open(path, "w") as fh:
+            fh.write("\n".join(lines) + "\n")

@@ -1,0 +1,202 @@
+"""Host-side driver of the PDAC hot path through its C-ABI.
+
+:class:`Solver` mirrors the reference time loop ``main_prog``
+(prog.f:143-329): it owns one context of a library that implements the
+``include/pdacgpu.h`` entry points, uploads a :class:`~pdac_amd.deck.Case`,
+and calls ``boundary``, ``fieldn`` and, per Runge-Kutta stage, ``tilde``,
+``iter``, ``ygas``, ``htilde`` and ``ftem`` in the reference's order.
+
+The class is library-agnostic (it is given a ``ctypes.CDLL`` and the symbol
+prefix) so that the same driver runs the HIP library and any other library
+that exports the same ABI.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import abi
+from .atmosphere import atmosphere_profile
+
+
+class PdacError(RuntimeError):
+    def __init__(self, where, code, msg=""):
+        super().__init__(f"{where} returned {code}{': ' + msg if msg else ''}")
+        self.code = code
+
+
+def bind(lib: ct.CDLL, prefix: str):
+    """Declare argument types of the C-ABI entry points on ``lib``."""
+    P = ct.c_void_p
+    dp, ip = ct.POINTER(ct.c_double), ct.POINTER(ct.c_int32)
+    sig = {
+        "destroy": [P], "set_grid": [P, dp, dp, dp, ip, dp, dp],
+        "set_params": [P, ct.POINTER(abi.Params)],
+        "put": [P, ct.c_int, dp, ct.c_int64], "get": [P, ct.c_int, dp, ct.c_int64],
+        "field_ncomp": [P, ct.c_int], "cnvert": [P], "boundary": [P], "fieldn": [P],
+        "begin_step": [P], "tilde": [P, ct.c_double],
+        "iter": [P, ct.c_double, ip, ip], "ygas": [P, ct.c_double],
+        "htilde": [P, ct.c_double], "ftem": [P, ct.c_double, ip],
+        "step": [P, ct.c_double, ct.c_int32, ct.c_int32, ip, ct.POINTER(ct.c_int64)],
+    }
+    out = {}
+    for name, args in sig.items():
+        fn = getattr(lib, prefix + name)
+        fn.argtypes = args
+        fn.restype = ct.c_int
+        out[name] = fn
+    return out
+
+
+class Solver:
+    """One PDAC domain driven through a C-ABI library.
+
+    ``create`` is a callable ``create(byref(ctx_ptr), byref(dims))`` that
+    allocates the context (the device entry point takes an extra device
+    index, the caller binds it)."""
+
+    def __init__(self, lib, prefix, create, case, precision=abi.PREC_FP64,
+                 iter_order=abi.ORDER_REDBLACK, p_atm=None, t_atm=None):
+        self.lib = lib
+        self.prefix = prefix
+        self.fn = bind(lib, prefix)
+        self.case = case
+        self.dims = abi.dims_from_case(case, precision)
+        self.params = abi.params_from_case(case, iter_order)
+        self.ctx = ct.c_void_p()
+        rc = create(ct.byref(self.ctx), ct.byref(self.dims))
+        if rc != 0:
+            raise PdacError(prefix + "create", rc, self.last_error())
+        self.n = case.ntot
+        dx, dy, dz = case.cell_sizes()
+        self._dx = np.ascontiguousarray(dx, dtype=np.float64)
+        self._dy = np.ascontiguousarray(dy if case.three_d else np.zeros(1), dtype=np.float64)
+        self._dz = np.ascontiguousarray(dz, dtype=np.float64)
+        self._flag = np.ascontiguousarray(case.flags(), dtype=np.int32)
+        if p_atm is None or t_atm is None:
+            p_atm, t_atm = atmosphere_profile(case)
+        self._patm = np.ascontiguousarray(p_atm, dtype=np.float64)
+        self._tatm = np.ascontiguousarray(t_atm, dtype=np.float64)
+        self._check("set_grid", self.fn["set_grid"](
+            self.ctx, abi.dptr(self._dx), abi.dptr(self._dy), abi.dptr(self._dz),
+            abi.iptr(self._flag), abi.dptr(self._patm), abi.dptr(self._tatm)))
+        self._check("set_params", self.fn["set_params"](self.ctx, ct.byref(self.params)))
+        self.time = case.time
+        self.nit_last = 0
+        self.nit_total = 0
+        if case.fields:
+            self.load_state(case.fields)
+
+    # ------------------------------------------------------------------
+    def last_error(self):
+        f = getattr(self.lib, "pdac_last_error", None)
+        if f is None:
+            return ""
+        f.restype = ct.c_char_p
+        msg = f()
+        return msg.decode() if msg else ""
+
+    def _check(self, where, rc):
+        if rc != 0:
+            raise PdacError(self.prefix + where, rc, self.last_error())
+        return rc
+
+    def close(self):
+        if self.ctx:
+            self.fn["destroy"](self.ctx)
+            self.ctx = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------
+    def ncomp(self, name):
+        return self.fn["field_ncomp"](self.ctx, abi.FIELD_ID[name])
+
+    def put(self, name, arr):
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).reshape(-1))
+        self._check("put(" + name + ")", self.fn["put"](self.ctx, abi.FIELD_ID[name], abi.dptr(a), a.size))
+
+    def get(self, name):
+        nc = self.ncomp(name)
+        a = np.empty(nc * self.n, dtype=np.float64)
+        self._check("get(" + name + ")", self.fn["get"](self.ctx, abi.FIELD_ID[name], abi.dptr(a), a.size))
+        return a.reshape(nc, self.n) if nc > 1 or name in (
+            "rlk", "ts", "sies", "us", "vs", "ws", "ygc", "xgc", "ck", "rlkn",
+            "siesn", "rusn", "rvsn", "rwsn", "rgpgcn", "rus", "rvs", "rws", "rhs",
+            "rsfe", "rsfn", "rsft", "appu", "appv", "appw") else a
+
+    def load_state(self, fields):
+        for name in abi.STATE_FIELDS:
+            if name in fields:
+                self.put(name, fields[name])
+        self._check("cnvert", self.fn["cnvert"](self.ctx))
+
+    def state(self, names=None):
+        names = names or [f for f in abi.STATE_FIELDS
+                          if self.case.three_d or f not in ("vg", "vs")]
+        return {nm: self.get(nm) for nm in names}
+
+    # ------------------------------------------------------------------
+    def boundary(self):
+        self._check("boundary", self.fn["boundary"](self.ctx))
+
+    def fieldn(self):
+        self._check("fieldn", self.fn["fieldn"](self.ctx))
+
+    def begin_step(self):
+        self._check("begin_step", self.fn["begin_step"](self.ctx))
+
+    def tilde(self, dt):
+        self._check("tilde", self.fn["tilde"](self.ctx, dt))
+
+    def iter(self, dt):
+        nit, conv = ct.c_int32(), ct.c_int32()
+        rc = self.fn["iter"](self.ctx, dt, ct.byref(nit), ct.byref(conv))
+        self._check("iter", rc)
+        return nit.value, bool(conv.value)
+
+    def ygas(self, dt):
+        self._check("ygas", self.fn["ygas"](self.ctx, dt))
+
+    def htilde(self, dt):
+        self._check("htilde", self.fn["htilde"](self.ctx, dt))
+
+    def ftem(self, dt):
+        fail = ct.c_int32()
+        self._check("ftem", self.fn["ftem"](self.ctx, dt, ct.byref(fail)))
+        return fail.value
+
+    def step(self, nsteps=1, dt=None, rungekut=None):
+        """Advance ``nsteps`` semi-implicit steps in one library call."""
+        dt = self.case.dt if dt is None else dt
+        rk = self.case.rungekut if rungekut is None else rungekut
+        nl, nt = ct.c_int32(), ct.c_int64()
+        self._check("step", self.fn["step"](self.ctx, dt, rk, nsteps, ct.byref(nl), ct.byref(nt)))
+        self.nit_last = nl.value
+        self.nit_total += nt.value
+        for _ in range(nsteps):
+            self.time = self.time + dt
+        return nl.value
+
+    def step_by_parts(self, dt=None, rungekut=None):
+        """One step as separate subroutine calls, in main_prog's order."""
+        dt = self.case.dt if dt is None else dt
+        rk_n = self.case.rungekut if rungekut is None else rungekut
+        self.boundary()
+        self.fieldn()
+        self.begin_step()
+        nit = 0
+        for rk in range(1, rk_n + 1):
+            dts = dt / (rk_n + 1 - rk)
+            self.tilde(dts)
+            nit, _ = self.iter(dts)
+            self.ygas(dts)
+            self.htilde(dts)
+            self.ftem(dts)
+        self.time = self.time + dt
+        return nit
