@@ -49,6 +49,7 @@ double pdac_cpu_sdrag(const pdac_params *prm, int is, double du, double dv,
 int pdac_cpu_caloric_eosg(const pdac_params *prm, int ngas, double sieg,
                           const double *ygc, double *tg, double *cg);
 void pdac_cpu_gauss_jordan(int np, const double *plim, double *a, double *b);
+int pdac_cpu_get_momflux(pdac_cpu_ctx *ctx, int comp, int face, double *out);
 
 #ifdef __cplusplus
 }

@@ -1,59 +1,81 @@
 """Atmospheric profile p_atm(k), t_atm(k) at cell centres.
 
-Restates set_atmosphere / compute_profile / hydrostatic
-(atmosphere.f:81-265).  Only free in/outflow boundaries read it.
+Restates set_atmosphere, compute_profile and hydrostatic
+(atmosphere.f:81-265) with the default layer table of input.f:433-446.
+Only free in/outflow boundaries read the profile.
 """
 from __future__ import annotations
+
+import math
 
 import numpy as np
 
 from . import constants as C
 
-# standard atmosphere layers (input.f:433-446 defaults)
-_LAYERS = [(1.1e4, -6.5e-3), (2.0e4, 0.0), (3.2e4, 1.0e-3), (4.7e4, 2.8e-3),
-           (5.1e4, 0.0), (7.1e4, -2.8e-3), (8.0e4, -2.0e-3)]
+# (top height [m], temperature gradient [K/m]) of the standard layers
+LAYERS = [(1.1e4, -6.5e-3), (2.0e4, 0.0), (3.2e4, 1.0e-3), (4.7e4, 2.8e-3),
+          (5.1e4, 0.0), (7.1e4, -2.8e-3), (8.0e4, -2.0e-3)]
 
 
-def _hydrostatic(zbot, ztop, gradt, tbot, pbot, gravz, rm):
-    """Temperature linear in z, hydrostatic pressure (atmosphere.f:217-265)."""
-    ttop = tbot + gradt * (ztop - zbot)
-    if gradt == 0.0:
-        ptop = pbot * np.exp(gravz / (rm * tbot) * (ztop - zbot))
+def hydrostatic(zb, z, grad, tb, pb, gravz):
+    """Linear temperature, hydrostatic pressure (atmosphere.f:239-263)."""
+    erreair = C.RGAS / C.GMW[5]
+    if grad != 0.0:
+        cost = gravz / (erreair * grad)
+        t = tb + grad * (z - zb)
+        p = pb * (t / tb) ** cost
     else:
-        ptop = pbot * (ttop / tbot) ** (gravz / (rm * gradt))
-    return ttop, ptop
+        cost = gravz / (erreair * tb)
+        t = tb
+        p = pb * math.exp(cost * (z - zb))
+    return t, p
+
+
+def cell_heights(dz, zzero=0.0):
+    """z(k) of grid_setup (grid.f:298-303)."""
+    nz = len(dz)
+    z = np.empty(nz)
+    zb = zzero
+    z[0] = -0.5 * dz[0] + zb
+    for k in range(1, nz):
+        zb = zb + dz[k]
+        z[k] = zb - 0.5 * dz[k]
+    return z
 
 
 def atmosphere_profile(case):
     """Return (p_atm, t_atm) over the nz cell rows of ``case``."""
-    nz = case.nz
-    dz = np.asarray(case.dz, dtype=np.float64)
-    z = np.empty(nz)
-    zb = 0.0
-    for k in range(nz):
-        if k == 0:
-            zb = 0.0
-            z[0] = -0.5 * dz[0] + zb
-        else:
-            zb = zb + dz[k]
-            z[k] = zb - 0.5 * dz[k]
-    if not case.stratification or case.gravz == 0.0:
-        return np.full(nz, case.p_ground), np.full(nz, case.t_ground)
-    rm = C.RGAS / C.GMW[5]  # air gas constant
-    layers = []
+    gravz = case.gravz
+    strat = case.stratification and gravz != 0.0  # control_atmosphere
     ptop, ttop = case.p_ground, case.t_ground
-    zbot = 0.0
-    for ztop, grad in _LAYERS:
-        tb, pb = ttop, ptop
-        ttop, ptop = _hydrostatic(zbot, ztop, grad, tb, pb, case.gravz, rm)
-        layers.append((zbot, ztop, grad, tb, pb))
-        zbot = ztop
+    tops = []
+    for l, (zt, grad) in enumerate(LAYERS):
+        zbot = 0.0 if l == 0 else LAYERS[l - 1][0]
+        pbot, tbot = ptop, ttop
+        ttop, ptop = hydrostatic(zbot, zt, grad, tbot, pbot, gravz)
+        tops.append((zt, ptop, ttop, grad))
+    z = cell_heights(np.asarray(case.dz, dtype=np.float64))
+    nz = len(z)
     p = np.empty(nz)
     t = np.empty(nz)
+    if z[0] <= tops[0][0]:
+        l = 0
+    else:
+        l = 1
     for k in range(nz):
         za = z[k]
-        for zb_, zt_, g_, tb_, pb_ in layers:
-            if za <= zt_ or zt_ == layers[-1][1]:
-                t[k], p[k] = _hydrostatic(zb_, max(za, zb_) if za >= 0 else za, g_, tb_, pb_, case.gravz, rm)
-                break
+        if za > 0.0:
+            if not strat:
+                zbot, pbot, tbot, grad = 0.0, case.p_ground, case.t_ground, 0.0
+            else:
+                if za > tops[l][0]:
+                    l += 1
+                if l == 0:
+                    zbot, pbot, tbot = 0.0, case.p_ground, case.t_ground
+                else:
+                    zbot, pbot, tbot = tops[l - 1][0], tops[l - 1][1], tops[l - 1][2]
+                grad = tops[l][3]
+            t[k], p[k] = hydrostatic(zbot, za, grad, tbot, pbot, gravz)
+        else:
+            t[k], p[k] = case.t_ground, case.p_ground
     return p, t

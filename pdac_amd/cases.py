@@ -78,7 +78,7 @@ def fill_state(case: Case, p, tg, ts, eps, ygc, ug=None, vg=None, wg=None,
 
 
 def shock_tube(nx=400, nz=8, dx=0.01, nsteps=1000, dt=1.0e-5, seed=1234,
-               nsolid=1, muscl=1, rungekut=3, **kw) -> Case:
+               nsolid=1, muscl=1, rungekut=3, bottom=3, top=2, **kw) -> Case:
     """2D Cartesian multiphase shock tube (nx includes the two wall columns).
 
     Left half: p = 1e6 Pa, T = 1000 K, one particle class (100 um, 2500 kg/m3)
@@ -107,3 +107,107 @@ def shock_tube(nx=400, nz=8, dx=0.01, nsteps=1000, dt=1.0e-5, seed=1234,
     # gas: 20 % water vapour (type 5), 80 % air (type 6) by mass
     ygc = np.stack([np.full(n, 0.2), np.full(n, 0.8)])
     return fill_state(case, p, tg, ts, eps, ygc)
+
+
+def _stretched(n, d0, d1, n0=1):
+    """Cell sizes growing geometrically from d0 to d1 (n cells, including the
+    boundary cell at index 0 which gets d0)."""
+    if n <= n0 + 1 or d1 == d0:
+        return np.full(n, float(d0))
+    r = (d1 / d0) ** (1.0 / (n - n0 - 1))
+    out = np.empty(n)
+    out[:n0] = d0
+    for i in range(n0, n):
+        out[i] = d0 * r ** (i - n0 + 1)
+    return np.minimum(out, d1)
+
+
+def eruption_column(nx=512, nz=1024, nsolid=2, seed=1234, dr0=1.0, dr1=20.0,
+                    vent_radius=50.0, w_vent=150.0, t_vent=1100.0, eps_vent=0.01,
+                    nsteps=5000, dt=None, top=2, east=4, **kw) -> Case:
+    """2D cylindrical collapsing eruption column.
+
+    Radial cells stretch from ``dr0`` at the axis to ``dr1``; vertical cells
+    likewise.  Stratified standard atmosphere (gravity -9.81), air + water
+    vapour, particle classes 50 and 500 um (2500 kg/m3).  The vent occupies
+    the lowest fluid row out to ``vent_radius``: gas w = ``w_vent`` with
+    seeded 2 % noise, T = ``t_vent``, particle fraction ``eps_vent`` split
+    evenly, water vapour only.  The axis is a slip wall, the ground no-slip,
+    the outer boundary free in/outflow; the top is a slip lid by default
+    (the reference's free in/outflow top reads a stale particle index,
+    bdry.f:561)."""
+    dx = _stretched(nx, dr0, dr1)
+    dz = _stretched(nz, dr0, dr1)
+    if dt is None:
+        dt = 0.2 * dr0 / (w_vent + 350.0)
+    case = Case(job_type="2D", itc=1, nx=nx, ny=1, nz=nz, dx=dx, dz=dz,
+                west=2, east=east, bottom=3, top=top,
+                nsolid=nsolid, diameter=[50.0, 500.0][:nsolid] + [100.0] * max(0, nsolid - 2),
+                density=[2500.0] * nsolid, sphericity=[1.0] * nsolid,
+                viscosity=[0.5] * nsolid, specific_heat=[1200.0] * nsolid,
+                thermal_conductivity=[2.0] * nsolid, twophase_limit=[1.0e-8] * nsolid,
+                gravz=-9.81, stratification=True, dt=dt, nsteps=nsteps, **kw)
+    from .atmosphere import atmosphere_profile
+    p_atm, t_atm = atmosphere_profile(case)
+    rng = np.random.default_rng(seed)
+    n = case.ntot
+    kk = np.repeat(np.arange(nz), nx)
+    ii = np.tile(np.arange(nx), nz)
+    xb = np.cumsum(dx) - dx[0]
+    r = xb - 0.5 * dx
+    p = p_atm[kk].copy()
+    tg = t_atm[kk].copy()
+    eps = np.full((nsolid, n), 0.0)
+    ygc = np.stack([np.zeros(n), np.ones(n)])
+    wg = np.zeros(n)
+    ws = np.zeros((nsolid, n))
+    vent = (kk == 1) & (ii >= 1) & (r[ii] <= vent_radius)
+    noise = rng.uniform(-1.0, 1.0, size=n)
+    wg[vent] = w_vent * (1.0 + 0.02 * noise[vent])
+    for s_ in range(nsolid):
+        ws[s_][vent] = wg[vent]
+        eps[s_][vent] = eps_vent / nsolid
+    tg[vent] = t_vent
+    ygc[0][vent] = 1.0
+    ygc[1][vent] = 0.0
+    ts = np.broadcast_to(tg, (nsolid, n)).copy()
+    return fill_state(case, p, tg, ts, eps, ygc, wg=wg, ws=ws)
+
+
+def mixed_box(three_d=False, nx=10, ny=7, nz=9, nsolid=2, seed=7, itc=0,
+              gravz=-9.81, nsteps=2, dt=2.0e-4, **kw) -> Case:
+    """Small randomised domain for parity tests: non-uniform cells, several
+    particle classes, seeded smooth random fields, walls of both kinds."""
+    rng = np.random.default_rng(seed)
+    dx = rng.uniform(0.8, 1.25, nx)
+    dz = rng.uniform(0.8, 1.25, nz)
+    dy = rng.uniform(0.8, 1.25, ny) if three_d else None
+    diam = list(rng.choice([30.0, 100.0, 250.0, 1000.0], size=nsolid, replace=False))
+    dens = list(rng.uniform(1000.0, 2700.0, nsolid))
+    case = Case(job_type="3D" if three_d else "2D", itc=0 if three_d else itc,
+                nx=nx, ny=ny if three_d else 1, nz=nz, dx=dx, dy=dy, dz=dz,
+                nsolid=nsolid, diameter=diam, density=dens,
+                sphericity=[1.0] * nsolid, viscosity=[0.5] * nsolid,
+                specific_heat=list(rng.uniform(900.0, 1500.0, nsolid)),
+                thermal_conductivity=[2.0] * nsolid, twophase_limit=[1.0e-8] * nsolid,
+                gravz=gravz, dt=dt, nsteps=nsteps, **kw)
+    n = case.ntot
+    shp = (nz, ny if three_d else 1, nx)
+    def smooth(a, b):
+        f = rng.uniform(a, b, size=shp)
+        for ax in range(3):
+            f = 0.5 * f + 0.25 * (np.roll(f, 1, ax) + np.roll(f, -1, ax))
+        return f.reshape(-1)
+    p = smooth(0.9e5, 1.3e5)
+    tg = smooth(300.0, 900.0)
+    eps = np.stack([smooth(0.0, 0.02) for _ in range(nsolid)])
+    ts = np.stack([tg + smooth(-50.0, 50.0) for _ in range(nsolid)])
+    y0 = smooth(0.0, 1.0)
+    ygc = np.stack([y0, 1.0 - y0])
+    vel = lambda: smooth(-20.0, 20.0)
+    kwv = dict(ug=vel(), wg=vel(), us=np.stack([vel() for _ in range(nsolid)]),
+               ws=np.stack([vel() for _ in range(nsolid)]))
+    if three_d:
+        kwv["vg"] = vel()
+        kwv["vs"] = np.stack([vel() for _ in range(nsolid)])
+    return fill_state(case, p, tg, ts, eps, ygc, **kwv)

@@ -139,6 +139,33 @@ class Case:
         return fl.reshape(-1)
 
     # ------------------------------------------------------------------
+    def scalars(self) -> dict:
+        """All non-field settings, JSON-serialisable."""
+        out = {}
+        for k, v in self.__dict__.items():
+            if k == "fields":
+                continue
+            if isinstance(v, np.ndarray):
+                v = v.tolist()
+            elif isinstance(v, (list, tuple)):
+                v = [float(x) if isinstance(x, (float, np.floating)) else
+                     (int(x) if isinstance(x, (int, np.integer)) else x) for x in v]
+            elif isinstance(v, np.generic):
+                v = v.item()
+            out[k] = v
+        return out
+
+    @classmethod
+    def from_scalars(cls, d: dict, fields=None) -> "Case":
+        kw = dict(d)
+        for k in ("dx", "dy", "dz"):
+            if kw.get(k) is not None:
+                kw[k] = np.asarray(kw[k], dtype=np.float64)
+        c = cls(**kw)
+        if fields:
+            c.fields = {k: np.array(v, dtype=np.float64) for k, v in fields.items()}
+        return c
+
     def restart_state(self) -> RestartState:
         st = RestartState(self.time, self.nx, self.ny, self.nz, self.nsolid,
                           len(self.gas_type), 0, list(self.gas_type),
@@ -220,10 +247,16 @@ class Case:
         lines.append(f" alphagrav = {d(self.alphagrav)}, ctu = 0, adapt_dt = F")
         lines.append("/")
         lines.append("'MESH'")
-        lines.append(" " + " ".join(d(x) for x in dx))
+
+        def rows(vals):
+            # a few values per record; list-directed input reads on
+            vals = [d(x) for x in vals]
+            return [" " + " ".join(vals[i:i + 4]) for i in range(0, len(vals), 4)]
+
+        lines += rows(dx)
         if self.three_d:
-            lines.append(" " + " ".join(d(x) for x in dy))
-        lines.append(" " + " ".join(d(x) for x in dz))
+            lines += rows(dy)
+        lines += rows(dz)
         lines.append("'FIXED_FLOWS'")
         lines.append(" 0")
         with open(path, "w") as fh:

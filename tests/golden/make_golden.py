@@ -1,0 +1,84 @@
+"""Write the golden fixtures of tests/golden/*.npz.
+
+Each fixture holds a small seeded case (its settings and initial fields) and
+the state the reference solver reached after ``nsteps`` steps, read from the
+restart file it wrote, plus the number of pressure-velocity sweeps it
+reported per step.  The reference executable is built in the build container
+from the reference's own sources by oracle/build_ref.sh; it never leaves it.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from pdac_amd.cases import eruption_column, mixed_box, shock_tube  # noqa: E402
+from tests.golden.refrun import available, run_reference  # noqa: E402
+
+CASES = {
+    # 2D Cartesian shock tube (slip walls), one class, MUSCL + minmod
+    "shock_tube_2d": lambda: shock_tube(nx=40, nz=8, nsteps=4, dt=1.0e-5, rungekut=1),
+    # the same with the 3-stage Runge-Kutta of prog.f:235-318
+    "shock_tube_2d_rk3": lambda: shock_tube(nx=40, nz=8, nsteps=2, dt=1.0e-5, rungekut=3),
+    # 2D Cartesian, 2 classes, gravity, non-uniform cells, no-slip ground
+    "mixed_2d": lambda: mixed_box(nsteps=3, bottom=3),
+    # 2D cylindrical, free in/outflow east boundary, stratified atmosphere
+    "mixed_2d_cyl_freeio": lambda: mixed_box(nsteps=3, itc=1, west=2, east=4, bottom=3,
+                                             top=2, stratification=True),
+    # Syamlal-O'Brien drag, pressure model A, Crank-Nicolson interphase terms
+    "mixed_2d_variants": lambda: mixed_box(nsteps=2, bottom=3, drag_model=2, pmodel=1,
+                                           alpha=0.5, alphagrav=0.5, rungekut=2),
+    # other limiters (van Leer mass, superbee velocity; not converging in 500)
+    "mixed_2d_limiters": lambda: mixed_box(nsteps=1, bottom=3, mass_limiter=1,
+                                           vel_limiter=3, beta=0.3),
+    # 3D, 3 classes, walls of both kinds and free in/outflow sides
+    "mixed_3d": lambda: mixed_box(three_d=True, nsteps=2, nsolid=3, bottom=3, west=3,
+                                  east=4, south=4, north=3, top=2, stratification=True),
+    # small eruption column: vent inflow, stratified atmosphere, 2 classes
+    "eruption_2d_small": lambda: eruption_column(nx=24, nz=40, nsteps=3, dt=2.0e-3,
+                                                 vent_radius=6.0),
+}
+
+
+def save(name, case, ref_state, nits):
+    data = {"settings": np.frombuffer(json.dumps(case.scalars()).encode(), dtype=np.uint8),
+            "nits": np.asarray(nits, dtype=np.int32)}
+    for k, v in case.fields.items():
+        data["in_" + k] = np.asarray(v)
+    for k, v in ref_state.fields.items():
+        data["out_" + k] = np.asarray(v)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **data)
+
+
+def load(name):
+    from pdac_amd.deck import Case
+    z = np.load(os.path.join(HERE, name + ".npz"))
+    settings = json.loads(bytes(z["settings"]).decode())
+    fields = {k[3:]: z[k] for k in z.files if k.startswith("in_")}
+    out = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
+    return Case.from_scalars(settings, fields), out, [int(x) for x in z["nits"]]
+
+
+def main(names=None):
+    if not available():
+        print("reference executable not built (oracle/build_ref.sh)")
+        return 1
+    for name, make in CASES.items():
+        if names and name not in names:
+            continue
+        case = make()
+        st, nits = run_reference(case)
+        save(name, case, st, nits)
+        print(f"{name}: {case.ntot} cells, {case.nsteps} steps, nit {nits}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -1,0 +1,136 @@
+"""Unit pins of the oracle's closed-form pieces against mathematics that
+does not come from retyping the reference: Sweby's limiter definitions, a
+library linear solve, the round trip of the caloric equation of state, the
+Stokes limit of the drag law and the restart file format."""
+import ctypes as ct
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle.binding import lib
+from pdac_amd import abi, constants as C
+from pdac_amd.cases import mixed_box
+from pdac_amd.restart import read_restart, write_restart
+
+
+def _lim(t, r, beta=0.25):
+    f = lib().pdac_cpu_limiter
+    f.restype = ct.c_double
+    f.argtypes = [ct.c_int, ct.c_double, ct.c_double]
+    return f(t, beta, r)
+
+
+R = np.linspace(-10, 10, 2001)
+
+
+def test_minmod_closed_form():
+    got = np.array([_lim(2, r) for r in R])
+    np.testing.assert_array_equal(got, np.clip(R, 0.0, 1.0))
+
+
+def test_vanleer_closed_form():
+    got = np.array([_lim(1, r) for r in R if r > -0.5])
+    rr = R[R > -0.5]
+    np.testing.assert_allclose(got, np.maximum(0.0, (rr + np.abs(rr)) / (1 + np.abs(rr))), rtol=1e-15, atol=1e-15)
+
+
+def test_superbee_closed_form():
+    got = np.array([_lim(3, r) for r in R])
+    ref = np.maximum.reduce([np.zeros_like(R), np.minimum(2 * R, 1), np.minimum(R, 2)])
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("t", [1, 2, 3])
+def test_tvd_region(t):
+    """Sweby's second-order TVD region: 0 <= psi <= min(2r, 2), psi(1) = 1."""
+    for r in R:
+        psi = _lim(t, r)
+        assert psi >= 0.0
+        if r > 0:
+            assert psi <= min(2 * r, 2.0) + 1e-15
+    assert _lim(t, 1.0) == 1.0
+
+
+def test_beta_limiter_and_first_order():
+    assert _lim(0, 0.3, beta=0.4) == (1 - 0.4) + 0.4 * 0.3
+    assert _lim(7, 0.3) == 0.0
+
+
+def test_gauss_jordan_against_numpy():
+    f = lib().pdac_cpu_gauss_jordan
+    f.argtypes = [ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]
+    rng = np.random.default_rng(3)
+    for np_ in (2, 3, 4, 5):
+        for _ in range(20):
+            a = rng.uniform(-1, 1, (np_, np_))
+            a = a + a.T + np.diag(np.abs(a).sum(1) * 2 + 1)
+            b = rng.uniform(-1, 1, np_)
+            x = np.linalg.solve(a, b)
+            aa, bb = a.copy(), b.copy()
+            plim = np.full(8, 1e-8)
+            f(np_, abi.dptr(plim), abi.dptr(aa), abi.dptr(bb))
+            np.testing.assert_allclose(bb, x, rtol=1e-12, atol=1e-14)
+
+
+def test_gauss_jordan_two_phase_collapse():
+    """A vanishing particle row folds its couplings into the diagonal and
+    zeroes that phase's velocity (matrix.f:353-362)."""
+    f = lib().pdac_cpu_gauss_jordan
+    a = np.array([[2.0, 0.5], [0.5, 1e-12]])
+    b = np.array([3.0, 7.0])
+    plim = np.full(8, 1e-8)
+    f(2, abi.dptr(plim), abi.dptr(a), abi.dptr(b))
+    assert b[1] == 0.0
+    assert b[0] == pytest.approx(3.0 / (2.0 + 0.5))
+
+
+def test_caloric_eos_round_trip():
+    f = lib().pdac_cpu_caloric_eosg
+    f.argtypes = [ct.POINTER(abi.Params), ct.c_int, ct.c_double, ct.POINTER(ct.c_double),
+                  ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]
+    case = mixed_box()
+    prm = abi.params_from_case(case)
+    y = np.array([0.3, 0.7])
+    for T in (250.0, 400.0, 1000.0, 1500.0):
+        cp = C.hcapg(T)
+        h = T * (cp[4] * y[0] + cp[5] * y[1])
+        tg = ct.c_double(300.0)
+        cg = ct.c_double(0.0)
+        info = f(ct.byref(prm), 2, h, abi.dptr(y), ct.byref(tg), ct.byref(cg))
+        assert info == 0
+        assert abs(tg.value - T) / T < 1e-7
+
+
+def test_drag_stokes_limit():
+    """Wen-Yu in the dilute regime is Stokes drag 18 mu eps / d^2 times the
+    Schiller-Naumann factor (1 + 0.15 Re^0.687) and the voidage function
+    ep^-2.65; at Re -> 0 the factor tends to 1."""
+    f = lib().pdac_cpu_kdrags
+    f.restype = ct.c_double
+    f.argtypes = [ct.POINTER(abi.Params), ct.c_int] + [ct.c_double] * 7
+    case = mixed_box(nsolid=1)
+    prm = abi.params_from_case(case)
+    mug, rgp, ep = 1.8e-5, 1.0, 0.99
+    rlk = (1 - ep) * prm.rl[0]
+    dk = prm.dk[0]
+    for re in (5e-3, 1e-1, 10.0):
+        du = re * mug / (rgp * dk)
+        k = f(ct.byref(prm), 0, du, 0.0, 0.0, ep, rgp, rlk, mug)
+        stokes = 18.0 * mug * (1 - ep) / dk ** 2 * ep ** float(np.float32(-2.65))
+        factor = 1 + float(np.float32(0.15)) * re ** float(np.float32(0.687))
+        assert k / stokes == pytest.approx(factor, rel=1e-9)
+    assert factor > 1.5  # the correction is exercised
+
+
+def test_restart_round_trip():
+    case = mixed_box(three_d=True, nsolid=2)
+    st = case.restart_state()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "pdac.res")
+        write_restart(path, st)
+        back = read_restart(path, True)
+    assert back.gas_type == st.gas_type and back.nx == st.nx
+    for k, v in st.fields.items():
+        np.testing.assert_array_equal(np.asarray(back.fields[k]).reshape(np.shape(v)), v)
