@@ -153,6 +153,11 @@ int pdac_step(pdac_ctx *ctx, double dt, int32_t rungekut, int32_t nsteps,
 int pdac_sync(pdac_ctx *ctx);
 const char *pdac_last_error(void);
 
+/* Diagnostic: momentum fluxes of the last pdac_tilde call (compute_all_
+ * fluxes_tilde, tilde.f:850-1326).  comp 0..2 = u,v,w; face 0..2 = e,n,t;
+ * out receives ntot*(nsolid+1) values, phase-major. */
+int pdac_get_momflux(pdac_ctx *ctx, int comp, int face, double *out);
+
 #ifdef __cplusplus
 }
 #endif

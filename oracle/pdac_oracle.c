@@ -339,9 +339,11 @@ static void neighbours(const pdac_cpu_ctx *s, int i, int j, int k, nbr *q) {
   q->bb = wallf(fl[q->ijkmm]) ? q->ijkm : q->ijkmm;
   if (k == 2) q->bb = q->b;
   if (s->three) {
-    q->nn = wallf(fl[q->ijppk]) ? q->ijpk : q->ijppk;
+    /* in 3D the y-direction fallbacks are the reflected first neighbours
+     * (ghost.f:1610-1622) */
+    q->nn = wallf(fl[q->ijppk]) ? q->n : q->ijppk;
     if (j == ny - 1) q->nn = q->n;
-    q->ss = wallf(fl[q->ijmmk]) ? q->ijmk : q->ijmmk;
+    q->ss = wallf(fl[q->ijmmk]) ? q->s_ : q->ijmmk;
     if (j == 2) q->ss = q->s_;
   } else {
     q->nn = q->ss = c;

@@ -1,0 +1,738 @@
+// pdac_kernels.hip -- gfx950 kernels of the PDAC semi-implicit per-step hot
+// path and the C-ABI of include/pdacgpu.h.
+//
+// One thread handles one cell (and, where the work is per phase, one
+// phase of one cell).  Writes are race-free by construction: a face flux is
+// produced by exactly one cell (the reference's "last writer", see each
+// kernel), and the pressure-velocity sweep runs in two colours so that a
+// cell's update writes only its own scalars and its own faces.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "pdac_dev.hpp"
+#include "pdac_flux.hpp"
+
+using namespace pdac;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(const char *where, hipError_t e) {
+  g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+  return -(int)e;
+}
+
+#define HIP_RET(call)                                        \
+  do {                                                       \
+    hipError_t e_ = (call);                                  \
+    if (e_ != hipSuccess) return set_error(#call, e_);       \
+  } while (0)
+
+constexpr int kBlock = 256;
+
+inline unsigned nblocks(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// decode a linear cell index into 1-based (i,j,k)
+__device__ __forceinline__ void decode(const Grid &g, long c, int &i, int &j, int &k) {
+  long nxy = (long)g.nx * g.ny;
+  k = (int)(c / nxy) + 1;
+  long r = c - (long)(k - 1) * nxy;
+  j = (int)(r / g.nx) + 1;
+  i = (int)(r - (long)(j - 1) * g.nx) + 1;
+}
+
+// wavefront-level reduction of a per-lane count, one atomic per wavefront
+__device__ __forceinline__ void wave_count_add(int32_t *ctr, bool pred) {
+  unsigned long long m = __ballot(pred);
+  if (m && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1))
+    atomicAdd(ctr, (int32_t)__popcll(m));
+}
+
+enum Ctr { C_DONE = 0, C_NOTCONV = 1, C_NIT = 2, C_IERR = 3, C_EOS = 4, C_N = 8 };
+
+}  // namespace
+
+// ======================================================================
+// boundary, bdry.f:22-726: each computed fluid cell fills the boundary
+// neighbours of its six faces in the reference's order (E, W, N, S, T, B).
+// Every boundary cell is face-adjacent to at most one computed cell in each
+// direction, so no two threads write the same value; each thread reads only
+// its own cell and interior cells no thread writes.
+template <int D, int NS>
+__device__ void inoutflow_d(const Grid &g, const Consts &K, Fields &F, double ugv, const double *usv,
+                            long n2, long n1, int k, double gnorm) {
+  long n = g.n;
+  int sgn = (n2 - n1) >= 0 ? 1 : -1;
+  double *tg = F.f[PDAC_TG], *sieg = F.f[PDAC_SIEG], *ygc = F.f[PDAC_YGC], *xgc = F.f[PDAC_XGC];
+  double *rog = F.f[PDAC_ROG], *rlk = F.f[PDAC_RLK], *ts = F.f[PDAC_TS], *sies = F.f[PDAC_SIES];
+  double *ep = F.f[PDAC_EP], *rgp = F.f[PDAC_RGP], *p = F.f[PDAC_P];
+  if (ugv * sgn > 0.0) {
+    tg[n2] = tg[n1];
+    sieg[n2] = sieg[n1];
+    for (int g_ = 0; g_ < K.ngas; ++g_) {
+      ygc[g_ * n + n2] = ygc[g_ * n + n1];
+      xgc[g_ * n + n2] = xgc[g_ * n + n1];
+    }
+    rog[n2] = rog[n1];
+  } else if (ugv * sgn <= 0.0) {
+    tg[n2] = g.t_atm[k];
+    double yl[PDAC_MAX_GAS], xl[PDAC_MAX_GAS];
+    for (int g_ = 0; g_ < K.ngas; ++g_) {
+      yl[g_] = K.atm_ygc[K.gas_type[g_] - 1];
+      ygc[g_ * n + n2] = yl[g_];
+    }
+    mole(K, yl, xl);
+    for (int g_ = 0; g_ < K.ngas; ++g_) xgc[g_ * n + n2] = xl[g_];
+    rog[n2] = thermal_eosg(K, g.t_atm[k], g.p_atm[k], xl);
+  }
+  double rls = 0.0;
+#pragma unroll
+  for (int is = 0; is < NS; ++is) {
+    if (usv[is] * sgn > 0.0) {
+      rlk[is * n + n2] = rlk[is * n + n1];
+      ts[is * n + n2] = ts[is * n + n1];
+      sies[is * n + n2] = sies[is * n + n1];
+    } else if (usv[is] * sgn <= 0.0) {
+      rlk[is * n + n2] = K.rl[is] * (1.0 - K.void_fraction) / (double)NS;
+      ts[is * n + n2] = g.t_atm[k];
+    }
+    rls = rls + rlk[is * n + n2] * K.inrl[is];
+  }
+  ep[n2] = 1.0 - rls;
+  rgp[n2] = rog[n2] * ep[n2];
+  double rhomix;
+  if (K.pmodel == 1) {
+    double sm = rlk[n2];
+    for (int is = 1; is < NS; ++is) sm = sm + rlk[is * n + n2];
+    rhomix = rgp[n2] + sm;
+  } else {
+    rhomix = rgp[n2];
+  }
+  p[n2] = p[n1] + rhomix * gnorm;
+}
+
+template <int NS>
+__device__ void zerograd_d(const Grid &g, const Consts &K, Fields &F, long n2, long n1) {
+  long n = g.n;
+  double *rlk = F.f[PDAC_RLK];
+  F.f[PDAC_TG][n2] = F.f[PDAC_TG][n1];
+  F.f[PDAC_SIEG][n2] = F.f[PDAC_SIEG][n1];
+  for (int g_ = 0; g_ < K.ngas; ++g_) {
+    F.f[PDAC_YGC][g_ * n + n2] = F.f[PDAC_YGC][g_ * n + n1];
+    F.f[PDAC_XGC][g_ * n + n2] = F.f[PDAC_XGC][g_ * n + n1];
+  }
+  F.f[PDAC_ROG][n2] = F.f[PDAC_ROG][n1];
+  double rls = 0.0;
+  for (int is = 0; is < NS; ++is) {
+    rlk[is * n + n2] = rlk[is * n + n1];
+    F.f[PDAC_TS][is * n + n2] = F.f[PDAC_TS][is * n + n1];
+    F.f[PDAC_SIES][is * n + n2] = F.f[PDAC_SIES][is * n + n1];
+    rls = rls + rlk[is * n + n2] * K.inrl[is];
+  }
+  F.f[PDAC_EP][n2] = 1.0 - rls;
+  F.f[PDAC_RGP][n2] = F.f[PDAC_ROG][n2] * F.f[PDAC_EP][n2];
+  F.f[PDAC_P][n2] = F.f[PDAC_P][n1];
+}
+
+__device__ __forceinline__ double extrapolate(double f2, double f1, double d12, double dx, int order) {
+  double gradient = (f2 - f1) / d12 * (double)order;
+  return f2 + gradient * dx;
+}
+
+template <int D, int NS>
+__global__ void __launch_bounds__(kBlock) k_boundary(Grid g, Consts K, Fields F) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.n) return;
+  const int32_t *fl = g.flag;
+  int f = fl[c];
+  if (!(f == FLUID || f == DOME_CELL || f == BL_CELL)) return;
+  int i, j, k;
+  decode(g, c, i, j, k);
+  long n = g.n;
+  double *ug = F.f[PDAC_UG], *vg = F.f[PDAC_VG], *wg = F.f[PDAC_WG];
+  double *us = F.f[PDAC_US], *vs = F.f[PDAC_VS], *ws = F.f[PDAC_WS];
+  long ipjk = rawn(g, i, j, k, 1, 0, 0), imjk = rawn(g, i, j, k, -1, 0, 0);
+  long ijkp = rawn(g, i, j, k, 0, 0, 1), ijkm = rawn(g, i, j, k, 0, 0, -1);
+  // quick exit for interior cells: every face neighbour is computed
+  bool inner = computed(fl[ipjk]) && computed(fl[imjk]) && computed(fl[ijkp]) && computed(fl[ijkm]);
+  long ijpk = c, ijmk = c;
+  if (D == 3) {
+    ijpk = rawn(g, i, j, k, 0, 1, 0);
+    ijmk = rawn(g, i, j, k, 0, -1, 0);
+    inner = inner && computed(fl[ijpk]) && computed(fl[ijmk]);
+  }
+  if (inner) return;
+  const long n1 = c;
+  double usv[NS];
+#define SETS(a, dst, src) _Pragma("unroll") for (int is_ = 0; is_ < NS; ++is_) a[is_ * n + (dst)] = (src)
+  // ---- east
+  {
+    long n2 = ipjk, n0 = imjk;
+    switch (fl[n2]) {
+    case SLIP_WALL:
+      ug[n1] = 0.0; SETS(us, n1, 0.0); ug[n2] = 0.0; SETS(us, n2, 0.0);
+      if (D == 3) { vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]); }
+      wg[n2] = wg[n1]; SETS(ws, n2, ws[is_ * n + n1]);
+      break;
+    case NOSLIP_WALL:
+      ug[n1] = 0.0; SETS(us, n1, 0.0); ug[n2] = 0.0; SETS(us, n2, 0.0);
+      if (!computed(fl[rawn(g, i, j, k, 1, 0, 1)])) { wg[n2] = -wg[n1]; SETS(ws, n2, -ws[is_ * n + n1]); }
+      if (D == 3 && !computed(fl[rawn(g, i, j, k, 1, 1, 0)])) { vg[n2] = -vg[n1]; SETS(vs, n2, -vs[is_ * n + n1]); }
+      break;
+    case FREE_IO: case ZERO_GRAD: {
+      ug[n1] = ug[n0]; SETS(us, n1, us[is_ * n + n0]);
+      ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]);
+      if (D == 3) { vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]); }
+      wg[n2] = wg[n1]; SETS(ws, n2, ws[is_ * n + n1]);
+      double gnorm = K.gravx * 0.5 * (g.dx[i + 1] + g.dx[i]);
+      if (fl[n2] == ZERO_GRAD) gnorm = 0.0;
+      for (int is = 0; is < NS; ++is) usv[is] = us[is * n + n1];
+      inoutflow_d<D, NS>(g, K, F, ug[n1], usv, n2, n1, k, gnorm);
+      break;
+    }
+    default: break;
+    }
+  }
+  // ---- west
+  {
+    long n2 = imjk;
+    switch (fl[n2]) {
+    case SLIP_WALL:
+      ug[n2] = 0.0; SETS(us, n2, 0.0);
+      if (D == 3) { vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]); }
+      wg[n2] = wg[n1]; SETS(ws, n2, ws[is_ * n + n1]);
+      break;
+    case NOSLIP_WALL: {
+      ug[n2] = 0.0; SETS(us, n2, 0.0);
+      bool ext = !computed(fl[rawn(g, i, j, k, -1, 0, 1)]);
+      if (ext) { wg[n2] = -wg[n1]; SETS(ws, n2, -ws[is_ * n + n1]); }
+      if (D == 3 && ext) { vg[n2] = -vg[n1]; SETS(vs, n2, -vs[is_ * n + n1]); }
+      break;
+    }
+    case FREE_IO: case ZERO_GRAD: {
+      ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]);
+      if (D == 3) { vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]); }
+      wg[n2] = wg[n1]; SETS(ws, n2, ws[is_ * n + n1]);
+      double gnorm = K.gravx * 0.5 * (g.dx[i - 1] + g.dx[i]);
+      if (fl[n2] == ZERO_GRAD) gnorm = 0.0;
+      for (int is = 0; is < NS; ++is) usv[is] = us[is * n + n2];
+      inoutflow_d<D, NS>(g, K, F, ug[n2], usv, n2, n1, k, gnorm);
+      break;
+    }
+    default: break;
+    }
+  }
+  if (D == 3) {
+    // ---- north
+    {
+      long n2 = ijpk, n0 = ijmk;
+      switch (fl[n2]) {
+      case SLIP_WALL:
+        vg[n1] = 0.0; SETS(vs, n1, 0.0); vg[n2] = 0.0; SETS(vs, n2, 0.0);
+        ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]);
+        wg[n2] = wg[n1]; SETS(ws, n2, ws[is_ * n + n1]);
+        break;
+      case NOSLIP_WALL:
+        vg[n1] = 0.0; SETS(vs, n1, 0.0); vg[n2] = 0.0; SETS(vs, n2, 0.0);
+        if (!computed(fl[rawn(g, i, j, k, 1, 1, 0)])) { ug[n2] = -ug[n1]; SETS(us, n2, -us[is_ * n + n1]); }
+        if (!computed(fl[rawn(g, i, j, k, 0, 1, 1)])) { wg[n2] = -wg[n1]; SETS(ws, n2, -ws[is_ * n + n1]); }
+        break;
+      case FREE_IO: case ZERO_GRAD: {
+        vg[n1] = vg[n0]; SETS(vs, n1, vs[is_ * n + n0]);
+        vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]);
+        ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]);
+        wg[n2] = wg[n1]; SETS(ws, n2, ws[is_ * n + n1]);
+        double gnorm = K.gravy * 0.5 * (g.dy[j] + g.dy[j + 1]);
+        if (fl[n2] == ZERO_GRAD) gnorm = 0.0;
+        for (int is = 0; is < NS; ++is) usv[is] = vs[is * n + n1];
+        inoutflow_d<D, NS>(g, K, F, vg[n1], usv, n2, n1, k, gnorm);
+        break;
+      }
+      default: break;
+      }
+    }
+    // ---- south
+    {
+      long n2 = ijmk;
+      switch (fl[n2]) {
+      case SLIP_WALL:
+        vg[n2] = 0.0; SETS(vs, n2, 0.0);
+        ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]);
+        wg[n2] = wg[n1]; SETS(ws, n2, ws[is_ * n + n1]);
+        break;
+      case NOSLIP_WALL:
+        vg[n2] = 0.0; SETS(vs, n2, 0.0);
+        if (!computed(fl[rawn(g, i, j, k, 1, -1, 0)])) { ug[n2] = -ug[n1]; SETS(us, n2, -us[is_ * n + n1]); }
+        if (!computed(fl[rawn(g, i, j, k, 0, -1, 1)])) { wg[n2] = -wg[n1]; SETS(ws, n2, -ws[is_ * n + n1]); }
+        break;
+      case FREE_IO: case ZERO_GRAD: {
+        vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]);
+        ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]);
+        wg[n2] = wg[n1]; SETS(ws, n2, ws[is_ * n + n1]);
+        double gnorm = K.gravy * 0.5 * (g.dy[j - 1] + g.dy[j]);
+        if (fl[n2] == ZERO_GRAD) gnorm = 0.0;
+        for (int is = 0; is < NS; ++is) usv[is] = vs[is * n + n2];
+        inoutflow_d<D, NS>(g, K, F, vg[n2], usv, n2, n1, k, gnorm);
+        break;
+      }
+      default: break;
+      }
+    }
+  }
+  // ---- top
+  {
+    long n2 = ijkp;
+    switch (fl[n2]) {
+    case SLIP_WALL:
+      wg[n1] = 0.0; SETS(ws, n1, 0.0);
+      ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]);
+      if (D == 3) { vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]); }
+      break;
+    case NOSLIP_WALL:
+      wg[n1] = 0.0; SETS(ws, n1, 0.0);
+      if (!computed(fl[rawn(g, i, j, k, 1, 0, 1)])) { ug[n2] = -ug[n1]; SETS(us, n2, -us[is_ * n + n1]); }
+      if (D == 3 && !computed(fl[rawn(g, i, j, k, 0, 1, 1)])) { vg[n2] = -vg[n1]; SETS(vs, n2, -vs[is_ * n + n1]); }
+      break;
+    case FREE_IO: case ZERO_GRAD: {
+      double d0 = g.dz[k - 1], d1 = g.dz[k], d2 = g.dz[k + 1];
+      long ijkmm = rawn(g, i, j, k, 0, 0, -2);
+      ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]);
+      if (D == 3) { vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]); }
+      wg[c] = extrapolate(wg[ijkm], wg[ijkmm], d0, d1, K.ord);
+      for (int is = 0; is < NS; ++is)
+        ws[is * n + c] = extrapolate(ws[is * n + ijkm], ws[is * n + ijkmm], d0, d1, K.ord);
+      wg[ijkp] = extrapolate(wg[c], wg[ijkm], d1, d2, K.ord);
+      for (int is = 0; is < NS; ++is)
+        ws[is * n + ijkp] = extrapolate(ws[is * n + c], ws[is * n + ijkm], d1, d2, K.ord);
+      if (fl[n2] == ZERO_GRAD) {
+        zerograd_d<NS>(g, K, F, n2, n1);
+      } else {
+        double gnorm = K.gravz * 0.5 * (g.dz[k] + g.dz[k + 1]);
+        for (int is = 0; is < NS; ++is) usv[is] = ws[is * n + n1];
+        inoutflow_d<D, NS>(g, K, F, wg[n1], usv, n2, n1, k + 1, gnorm);
+      }
+      break;
+    }
+    default: break;
+    }
+  }
+  // ---- bottom
+  {
+    long n2 = ijkm;
+    switch (fl[n2]) {
+    case SLIP_WALL:
+      wg[n2] = 0.0; SETS(ws, n2, 0.0);
+      if (fl[rawn(g, i, j, k, 1, 0, -1)] == SLIP_WALL) { ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]); }
+      if (D == 3 && fl[rawn(g, i, j, k, 0, 1, -1)] == SLIP_WALL) { vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]); }
+      break;
+    case NOSLIP_WALL:
+      wg[n2] = 0.0; SETS(ws, n2, 0.0);
+      if (fl[rawn(g, i, j, k, 1, 0, -1)] == NOSLIP_WALL) { ug[n2] = -ug[n1]; SETS(us, n2, -us[is_ * n + n1]); }
+      if (D == 3 && fl[rawn(g, i, j, k, 0, 1, -1)] == NOSLIP_WALL) { vg[n2] = -vg[n1]; SETS(vs, n2, -vs[is_ * n + n1]); }
+      break;
+    case FREE_IO: case ZERO_GRAD: {
+      wg[n2] = wg[n1]; SETS(ws, n2, ws[is_ * n + n1]);
+      ug[n2] = ug[n1]; SETS(us, n2, us[is_ * n + n1]);
+      if (D == 3) { vg[n2] = vg[n1]; SETS(vs, n2, vs[is_ * n + n1]); }
+      double gnorm = K.gravz * 0.5 * (g.dz[k - 1] + g.dz[k]);
+      if (fl[n2] == ZERO_GRAD) gnorm = 0.0;
+      for (int is = 0; is < NS; ++is) usv[is] = ws[is * n + n2];
+      inoutflow_d<D, NS>(g, K, F, wg[n2], usv, n2, n1, k - 1, gnorm);
+      break;
+    }
+    default: break;
+    }
+  }
+#undef SETS
+}
+
+// ======================================================================
+// fieldn, tilde.f:125-274: step-start copies, face momenta, viscosity
+template <int D, int NS>
+__global__ void __launch_bounds__(kBlock) k_fieldn(Grid g, Consts K, Fields F) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.n) return;
+  long n = g.n;
+  const double *rgp = F.f[PDAC_RGP], *rlk = F.f[PDAC_RLK];
+  if (computed(g.flag[c])) {
+    int i, j, k;
+    decode(g, c, i, j, k);
+    Nbr<D> q;
+    first_nbrs<D>(g, i, j, k, q);
+    const double *dx = g.dx, *dy = g.dy, *dz = g.dz;
+    double indxp = 1.0 / (dx[i] + dx[i + 1]), indzp = 1.0 / (dz[k] + dz[k + 1]);
+    double rgp_e = (dx[i + 1] * rgp[c] + dx[i] * rgp[q.e]) * indxp;
+    double rgp_t = (dz[k + 1] * rgp[c] + dz[k] * rgp[q.t]) * indzp;
+    F.f[PDAC_RUGN][c] = rgp_e * F.f[PDAC_UG][c];
+    F.f[PDAC_RWGN][c] = rgp_t * F.f[PDAC_WG][c];
+    double indyp = 0.0;
+    if (D == 3) {
+      indyp = 1.0 / (dy[j] + dy[j + 1]);
+      double rgp_n = (dy[j + 1] * rgp[c] + dy[j] * rgp[q.n]) * indyp;
+      F.f[PDAC_RVGN][c] = rgp_n * F.f[PDAC_VG][c];
+    }
+#pragma unroll
+    for (int is = 0; is < NS; ++is) {
+      const double *rl = rlk + is * n;
+      double rlk_e = (rl[c] * dx[i + 1] + rl[q.e] * dx[i]) * indxp;
+      double rlk_t = (rl[c] * dz[k + 1] + rl[q.t] * dz[k]) * indzp;
+      F.f[PDAC_RUSN][is * n + c] = rlk_e * F.f[PDAC_US][is * n + c];
+      F.f[PDAC_RWSN][is * n + c] = rlk_t * F.f[PDAC_WS][is * n + c];
+      if (D == 3) {
+        double rlk_n = (rl[c] * dy[j + 1] + rl[q.n] * dy[j]) * indyp;
+        F.f[PDAC_RVSN][is * n + c] = rlk_n * F.f[PDAC_VS][is * n + c];
+      }
+    }
+  }
+  F.f[PDAC_PN][c] = F.f[PDAC_P][c];
+  F.f[PDAC_EPN][c] = F.f[PDAC_EP][c];
+  double rgpn = rgp[c];
+  F.f[PDAC_RGPN][c] = rgpn;
+  F.f[PDAC_SIEGN][c] = F.f[PDAC_SIEG][c];
+  double xl[PDAC_MAX_GAS];
+  for (int g_ = 0; g_ < K.ngas; ++g_) {
+    F.f[PDAC_RGPGCN][g_ * n + c] = rgpn * F.f[PDAC_YGC][g_ * n + c];
+    xl[g_] = F.f[PDAC_XGC][g_ * n + c];
+  }
+#pragma unroll
+  for (int is = 0; is < NS; ++is) {
+    F.f[PDAC_RLKN][is * n + c] = rlk[is * n + c];
+    F.f[PDAC_SIESN][is * n + c] = F.f[PDAC_SIES][is * n + c];
+  }
+  double mu, kap;
+  viscon(K, xl, F.f[PDAC_TG][c], mu, kap);
+  F.f[PDAC_MUG][c] = mu;
+  F.f[PDAC_KAPG][c] = kap;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill(double *a, long n, double v) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) a[c] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy(double *dst, const double *src, long n) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) dst[c] = src[c];
+}
+
+// ======================================================================
+// tilde (part 1), compute_all_fluxes_tilde, tilde.f:850-1326: momentum
+// fluxes of one phase (blockIdx.y) of each computed cell.  E/N/T faces are
+// owned by the cell; a W/S/B face is written only when that neighbour is
+// not computed (then no other cell writes it).
+template <int D, int NS>
+__global__ void __launch_bounds__(kBlock) k_momflux(Grid g, Consts K, Fields F) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.n) return;
+  int fc = g.flag[c];
+  if (!computed(fc)) return;
+  int ph = blockIdx.y;
+  long n = g.n;
+  int i, j, k;
+  decode(g, c, i, j, k);
+  FullIdx<D> x;
+  full_idx<D>(g, i, j, k, x);
+  const double *uu, *vv, *ww, *dd;
+  if (ph == 0) { uu = F.f[PDAC_UG]; vv = F.f[PDAC_VG]; ww = F.f[PDAC_WG]; dd = F.f[PDAC_RGP]; }
+  else {
+    long o = (long)(ph - 1) * n;
+    uu = F.f[PDAC_US] + o; vv = F.f[PDAC_VS] + o; ww = F.f[PDAC_WS] + o; dd = F.f[PDAC_RLK] + o;
+  }
+  Stc u, v, w, dens, sx, sy, sz;
+  gather_rnb<D>(x, uu, u);
+  gather_rnb<D>(x, ww, w);
+  if (D == 3) gather_rnb<D>(x, vv, v);
+  gather_nb<D>(x, dd, dens);
+  interp_x<D>(g, dens, sx, i);
+  interp_z<D>(g, dens, sz, k);
+  if (D == 3) interp_y(g, dens, sy, j);
+  const int32_t *fl = g.flag;
+  bool cw = computed(fl[x.w]), cb = computed(fl[x.b]);
+  bool cs_ = (D == 3) ? computed(fl[x.s]) : true;
+  bool mus = K.muscl > 0 && fc == FLUID;
+  long o = (long)ph * n;
+  Faces fu, fv, fw;
+  flux_u<D>(g, K, cw, cs_, cb, sx, u, v, w, i, j, k, mus && i != g.nx - 1, fu);
+  flux_w<D>(g, K, cw, cs_, cb, sz, u, v, w, i, j, k, mus && k != g.nz - 1, fw);
+  F.mf[0][0][o + c] = fu.fe; F.mf[0][2][o + c] = fu.ft;
+  F.mf[2][0][o + c] = fw.fe; F.mf[2][2][o + c] = fw.ft;
+  if (fu.has_w) { F.mf[0][0][o + x.w] = fu.fw; F.mf[2][0][o + x.w] = fw.fw; }
+  if (fu.has_b) { F.mf[0][2][o + x.b] = fu.fb; F.mf[2][2][o + x.b] = fw.fb; }
+  if (D == 3) {
+    flux_v(g, K, cw, cs_, cb, sy, u, v, w, i, j, k, mus && j != g.ny - 1, fv);
+    F.mf[0][1][o + c] = fu.fn; F.mf[1][0][o + c] = fv.fe; F.mf[1][1][o + c] = fv.fn;
+    F.mf[1][2][o + c] = fv.ft; F.mf[2][1][o + c] = fw.fn;
+    if (fu.has_w) F.mf[1][0][o + x.w] = fv.fw;
+    if (fu.has_b) F.mf[1][2][o + x.b] = fv.fb;
+    if (fu.has_s) { F.mf[0][1][o + x.s] = fu.fs; F.mf[1][1][o + x.s] = fv.fs; F.mf[2][1][o + x.s] = fw.fs; }
+  }
+}
+
+// inter, drag.f:173-246: packed symmetric interphase matrix with zero row sums
+template <int NS>
+__device__ __forceinline__ void inter_d(const Consts &K, const double *kpgv, double dt,
+                                        const double *rlkc, const double (&du_pp)[NS][NS],
+                                        const double (&dv_pp)[NS][NS], const double (&dw_pp)[NS][NS],
+                                        double *au, double *av, double *aw) {
+  constexpr int NP = NS + 1;
+  int ks = 0;
+#pragma unroll
+  for (int k = 1; k <= NP; ++k) {
+    if (k == 1) {
+      ks = ks + 1;
+      au[ks - 1] = 0.0; av[ks - 1] = 0.0; aw[ks - 1] = 0.0;
+    } else {
+#pragma unroll
+      for (int kk = 1; kk <= k; ++kk) {
+        ks = ks + 1;
+        if (kk == 1) {
+          au[ks - 1] = -kpgv[k - 2] * dt; av[ks - 1] = -kpgv[k - 2] * dt; aw[ks - 1] = -kpgv[k - 2] * dt;
+        } else if (kk == k) {
+          au[ks - 1] = 0.0; av[ks - 1] = 0.0; aw[ks - 1] = 0.0;
+        } else {
+          double con = ppdrag(K, rlkc, k, kk);
+          au[ks - 1] = -(con * du_pp[k - 2][kk - 2]) * dt;
+          av[ks - 1] = -(con * dv_pp[k - 2][kk - 2]) * dt;
+          aw[ks - 1] = -(con * dw_pp[k - 2][kk - 2]) * dt;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 1; k <= NP; ++k) {
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+    for (int kk = 1; kk <= k; ++kk) {
+      int i_ = kk + k * (k - 1) / 2 - 1;
+      sx = sx + au[i_]; sy = sy + av[i_]; sz = sz + aw[i_];
+    }
+#pragma unroll
+    for (int kk = k; kk <= NP; ++kk) {
+      int i_ = k + kk * (kk - 1) / 2 - 1;
+      sx = sx + au[i_]; sy = sy + av[i_]; sz = sz + aw[i_];
+    }
+    int d = k * (k + 1) / 2 - 1;
+    au[d] = -sx; av[d] = -sy; aw[d] = -sz;
+  }
+}
+
+// tilde (part 2), mesh_loop of tilde.f:418-806: explicit momenta, drag,
+// interphase matrix and the (1-alpha) coupling.  Neighbour interphase
+// entries come from the copy taken before this call (the serial loop reads
+// east/north/top neighbours before it reaches them).
+template <int D, int NS>
+__global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, double dt, int old_zero) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.n) return;
+  if (!computed(g.flag[c])) return;
+  constexpr int NP = NS + 1;
+  constexpr int NA = NP * (NP + 1) / 2;
+  long n = g.n;
+  int i, j, k;
+  decode(g, c, i, j, k);
+  Nbr<D> q;
+  first_nbrs<D>(g, i, j, k, q);
+  double *const(*mf)[3] = F.mf;
+  const double *dx = g.dx, *dy = g.dy, *dz = g.dz;
+  const double *rgpn = F.f[PDAC_RGPN], *rlkn = F.f[PDAC_RLKN], *pn = F.f[PDAC_PN], *epn = F.f[PDAC_EPN];
+  const double *ug = F.f[PDAC_UG], *vg = F.f[PDAC_VG], *wg = F.f[PDAC_WG];
+  const double *us = F.f[PDAC_US], *vs = F.f[PDAC_VS], *ws = F.f[PDAC_WS];
+  double ugfx = mf[0][0][c] - mf[0][0][q.imjk];
+  double ugfz = mf[0][2][c] - mf[0][2][q.ijkm];
+  double wgfx = mf[2][0][c] - mf[2][0][q.imjk];
+  double wgfz = mf[2][2][c] - mf[2][2][q.ijkm];
+  double ugfy = 0.0, vgfx = 0.0, vgfy = 0.0, vgfz = 0.0, wgfy = 0.0;
+  if (D == 3) {
+    ugfy = mf[0][1][c] - mf[0][1][q.ijmk];
+    vgfx = mf[1][0][c] - mf[1][0][q.imjk];
+    vgfy = mf[1][1][c] - mf[1][1][q.ijmk];
+    vgfz = mf[1][2][c] - mf[1][2][q.ijkm];
+    wgfy = mf[2][1][c] - mf[2][1][q.ijmk];
+  }
+  double dxi = dx[i], dxip1 = dx[i + 1], dyj = dy[j], dyjp1 = dy[j + 1];
+  double dzk = dz[k], dzkp1 = dz[k + 1];
+  double indxp = 1.0 / (dx[i] + dx[i + 1]), indyp = 1.0 / (dy[j] + dy[j + 1]);
+  double indzp = 1.0 / (dz[k] + dz[k + 1]);
+  const double alpha = K.alpha, ag = K.alphagrav;
+  double epn_e, epn_n, epn_t;
+  if (K.pmodel == 1) {
+    epn_e = (dxi * epn[q.e] + dxip1 * epn[c]) * indxp;
+    epn_n = (dyj * epn[q.n] + dyjp1 * epn[c]) * indyp;
+    epn_t = (dzk * epn[q.t] + dzkp1 * epn[c]) * indzp;
+  } else {
+    epn_e = epn_n = epn_t = 1.0;
+  }
+  double t, rug, rvg = 0.0, rwg;
+  t = 0.0;
+  t = t - indxp * 2.0 * ugfx * g.inrb[i];
+  t = t + (1.0 - ag) * (dx[i + 1] * rgpn[c] + dx[i] * rgpn[q.e]) * indxp * K.gravx;
+  t = t - g.indy[j] * ugfy;
+  t = t - g.indz[k] * ugfz;
+  t = t + (1.0 - alpha) * indxp * 2.0 * epn_e * (pn[c] - pn[q.e]);
+  rug = F.f[PDAC_RUGN][c] + dt * t;
+  if (D == 3) {
+    t = 0.0;
+    t = t + (1.0 - ag) * (dy[j + 1] * rgpn[c] + dy[j] * rgpn[q.n]) * indyp * K.gravy;
+    t = t - g.indx[i] * vgfx;
+    t = t - indyp * 2.0 * vgfy;
+    t = t - g.indz[k] * vgfz;
+    t = t + (1.0 - alpha) * indyp * 2.0 * epn_n * (pn[c] - pn[q.n]);
+    rvg = F.f[PDAC_RVGN][c] + dt * t;
+  }
+  t = 0.0;
+  t = t + (1.0 - ag) * (dz[k + 1] * rgpn[c] + dz[k] * rgpn[q.t]) * indzp * K.gravz;
+  t = t - g.indx[i] * wgfx * g.inr[i];
+  t = t - g.indy[j] * wgfy;
+  t = t - indzp * 2.0 * wgfz;
+  t = t + (1.0 - alpha) * indzp * 2.0 * epn_t * (pn[c] - pn[q.t]);
+  rwg = F.f[PDAC_RWGN][c] + dt * t;
+  double rus[NS], rvs[NS], rws[NS], kpgv[NS], rlkc[NS];
+  const double *rgp = F.f[PDAC_RGP], *rlk = F.f[PDAC_RLK], *ep = F.f[PDAC_EP], *mug = F.f[PDAC_MUG];
+#pragma unroll
+  for (int is = 0; is < NS; ++is) {
+    long o = (long)(is + 1) * n, os = (long)is * n;
+    rlkc[is] = rlk[os + c];
+    double usfx = mf[0][0][o + c] - mf[0][0][o + q.imjk];
+    double usfz = mf[0][2][o + c] - mf[0][2][o + q.ijkm];
+    double wsfx = mf[2][0][o + c] - mf[2][0][o + q.imjk];
+    double wsfz = mf[2][2][o + c] - mf[2][2][o + q.ijkm];
+    double usfy = 0.0, vsfx = 0.0, vsfy = 0.0, vsfz = 0.0, wsfy = 0.0;
+    if (D == 3) {
+      usfy = mf[0][1][o + c] - mf[0][1][o + q.ijmk];
+      vsfx = mf[1][0][o + c] - mf[1][0][o + q.imjk];
+      vsfy = mf[1][1][o + c] - mf[1][1][o + q.ijmk];
+      vsfz = mf[1][2][o + c] - mf[1][2][o + q.ijkm];
+      wsfy = mf[2][1][o + c] - mf[2][1][o + q.ijmk];
+    }
+    double epsn_e, epsn_n, epsn_t;
+    if (K.pmodel == 1) {
+      epsn_e = (dxi * rlkn[os + q.e] + dxip1 * rlkn[os + c]) * indxp * K.inrl[is];
+      epsn_n = (dyj * rlkn[os + q.n] + dyjp1 * rlkn[os + c]) * indyp * K.inrl[is];
+      epsn_t = (dzk * rlkn[os + q.t] + dzkp1 * rlkn[os + c]) * indzp * K.inrl[is];
+    } else {
+      epsn_e = epsn_n = epsn_t = 0.0;
+    }
+    const double sink = 0.0;
+    t = 0.0;
+    t = t - indxp * 2.0 * usfx * g.inrb[i];
+    t = t + (1.0 - ag) * indxp * K.gravx * (dx[i + 1] * rlkn[os + c] + dx[i] * rlkn[os + q.e]);
+    t = t - g.indy[j] * usfy;
+    t = t - g.indz[k] * usfz;
+    t = t + (1.0 - alpha) * indxp * 2.0 * epsn_e * (pn[c] - pn[q.e]);
+    rus[is] = F.f[PDAC_RUSN][os + c] + dt * t;
+    rus[is] = rus[is] + dt * sink * us[os + c];
+    if (D == 3) {
+      t = 0.0;
+      t = t + (1.0 - ag) * indyp * K.gravy * (dy[j + 1] * rlkn[os + c] + dy[j] * rlkn[os + q.n]);
+      t = t - g.indx[i] * vsfx;
+      t = t - indyp * 2.0 * vsfy;
+      t = t - g.indz[k] * vsfz;
+      t = t + (1.0 - alpha) * indyp * 2.0 * epsn_n * (pn[c] - pn[q.n]);
+      rvs[is] = F.f[PDAC_RVSN][os + c] + dt * t;
+      rvs[is] = rvs[is] + dt * sink * vs[os + c];
+    } else {
+      rvs[is] = 0.0;
+    }
+    t = 0.0;
+    t = t + (1.0 - ag) * indzp * K.gravz * (rlkn[os + c] * dz[k + 1] + rlkn[os + q.t] * dz[k]);
+    t = t - g.indx[i] * wsfx * g.inr[i];
+    t = t - g.indy[j] * wsfy;
+    t = t - indzp * 2.0 * wsfz;
+    t = t + (1.0 - alpha) * indzp * 2.0 * epsn_t * (pn[c] - pn[q.t]);
+    rws[is] = F.f[PDAC_RWSN][os + c] + dt * t;
+    rws[is] = rws[is] + dt * sink * ws[os + c];
+    double dugs = ((ug[c] - us[os + c]) + (ug[q.imjk] - us[os + q.imjk])) * 0.5;
+    double dwgs = ((wg[c] - ws[os + c]) + (wg[q.ijkm] - ws[os + q.ijkm])) * 0.5;
+    double dvgs = (D == 3) ? ((vg[c] - vs[os + c]) + (vg[q.ijmk] - vs[os + q.ijmk])) * 0.5 : 0.0;
+    if (K.drag_model == 1) kpgv[is] = kdrags(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c]);
+    else kpgv[is] = sdrag(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c]);
+  }
+  // relative particle-particle velocities at the cell centre (drag.f:216-218)
+  double du_pp[NS][NS], dv_pp[NS][NS], dw_pp[NS][NS];
+#pragma unroll
+  for (int a = 0; a < NS; ++a)
+#pragma unroll
+    for (int b = 0; b < NS; ++b) {
+      long oa = (long)a * n, ob = (long)b * n;
+      du_pp[a][b] = fabs(us[oa + c] - us[ob + c] + us[oa + q.imjk] - us[ob + q.imjk]) * 0.5;
+      dw_pp[a][b] = fabs(ws[oa + c] - ws[ob + c] + ws[oa + q.ijkm] - ws[ob + q.ijkm]) * 0.5;
+      dv_pp[a][b] = (D == 3) ? fabs(vs[oa + c] - vs[ob + c] + vs[oa + q.ijmk] - vs[ob + q.ijmk]) * 0.5 : 0.0;
+    }
+  double au[NA], av[NA], aw[NA];
+  inter_d<NS>(K, kpgv, dt, rlkc, du_pp, dv_pp, dw_pp, au, av, aw);
+  double *appu = F.f[PDAC_APPU], *appv = F.f[PDAC_APPV], *appw = F.f[PDAC_APPW];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    appu[(long)a * n + c] = au[a];
+    appw[(long)a * n + c] = aw[a];
+    if (D == 3) appv[(long)a * n + c] = av[a];
+  }
+  {
+    // (1-alpha) interphase coupling, tilde.f:637-706.  The serial loop reads
+    // the east/north/top entries before it reaches those cells: they hold the
+    // previous call's values (zero after allocate_fluxes), unless the
+    // neighbour is the cell itself.
+    const double *ou = F.appu_old[0], *ov = F.appu_old[1], *ow = F.appu_old[2];
+    auto AE = [&](int ls) { return q.e == c ? au[ls] : (old_zero ? 0.0 : ou[(long)ls * n + q.e]); };
+    auto AN = [&](int ls) { return q.n == c ? av[ls] : (old_zero ? 0.0 : ov[(long)ls * n + q.n]); };
+    auto AT = [&](int ls) { return q.t == c ? aw[ls] : (old_zero ? 0.0 : ow[(long)ls * n + q.t]); };
+    auto CNX = [&](int ls) { return (1.0 - alpha) * (dxi * AE(ls) + dxip1 * au[ls]) * indxp; };
+    auto CNY = [&](int ls) { return (1.0 - alpha) * (dyj * AN(ls) + dyjp1 * av[ls]) * indyp; };
+    auto CNZ = [&](int ls) { return (1.0 - alpha) * (dzk * AT(ls) + dzkp1 * aw[ls]) * indzp; };
+    rug = rug - CNX(0) * ug[c];
+    if (D == 3) rvg = rvg - CNY(0) * vg[c];
+    rwg = rwg - CNZ(0) * wg[c];
+#pragma unroll
+    for (int l = 2; l <= NP; ++l) {
+      int ls1 = l * (l - 1) / 2;
+      long ol = (long)(l - 2) * n;
+#pragma unroll
+      for (int ll = 1; ll <= l; ++ll) {
+        int ls = ls1 + ll - 1;
+        if (ll == 1) {
+          rug = rug - CNX(ls) * us[ol + c];
+          if (D == 3) rvg = rvg - CNY(ls) * vs[ol + c];
+          rwg = rwg - CNZ(ls) * ws[ol + c];
+          rus[l - 2] = rus[l - 2] - CNX(ls) * ug[c];
+          if (D == 3) rvs[l - 2] = rvs[l - 2] - CNY(ls) * vg[c];
+          rws[l - 2] = rws[l - 2] - CNZ(ls) * wg[c];
+        } else if (ll == l) {
+          rus[l - 2] = rus[l - 2] - CNX(ls) * us[ol + c];
+          if (D == 3) rvs[l - 2] = rvs[l - 2] - CNY(ls) * vs[ol + c];
+          rws[l - 2] = rws[l - 2] - CNZ(ls) * ws[ol + c];
+        } else {
+          long oll = (long)(ll - 2) * n;
+          rus[l - 2] = rus[l - 2] - CNX(ls) * us[oll + c];
+          rus[ll - 2] = rus[ll - 2] - CNX(ls) * us[ol + c];
+          if (D == 3) {
+            rvs[l - 2] = rvs[l - 2] - CNY(ls) * vs[oll + c];
+            rvs[ll - 2] = rvs[ll - 2] - CNY(ls) * vs[ol + c];
+          }
+          rws[l - 2] = rws[l - 2] - CNZ(ls) * ws[oll + c];
+          rws[ll - 2] = rws[ll - 2] - CNZ(ls) * ws[ol + c];
+        }
+      }
+    }
+  }
+  F.f[PDAC_RUG][c] = rug;
+  F.f[PDAC_RWG][c] = rwg;
+  if (D == 3) F.f[PDAC_RVG][c] = rvg;
+#pragma unroll
+  for (int is = 0; is < NS; ++is) {
+    F.f[PDAC_RUS][(long)is * n + c] = rus[is];
+    F.f[PDAC_RWS][(long)is * n + c] = rws[is];
+    if (D == 3) F.f[PDAC_RVS][(long)is * n + c] = rvs[is];
+  }
+}
+
+#include "pdac_iter.hip.inc"
+#include "pdac_energy.hip.inc"
+#include "pdac_api.hip.inc"
