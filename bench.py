@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Throughput of the PDAC semi-implicit step on one MI355X.
+
+Workload: the 2D cylindrical collapsing eruption column, 512 radial x 1024
+vertical cells (1 m at the axis stretching to 20 m), 2 particle classes
+(50 and 500 um), air + water vapour, stratified atmosphere, axial vent of
+50 m radius at w = 150 m/s with seeded 2 % noise; plain FP64, rungekut = 1,
+MUSCL + minmod.  Synthetic seeded initial state, resident in HBM before the
+timed region.
+
+One step is main_prog's step (prog.f:143-329): boundary, fieldn, tilde,
+iter (red-black sweeps until every cell's mass residual is below
+delg*rgp), ygas, htilde, ftem.  The metric is Mcell-steps/s over the whole
+mesh including boundary cells.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line with a "roofline" object for the dominant kernel
+(algorithmic bytes per launch over its mean HIP-event duration) and a
+"cpu_baseline" object (the CPU restatement of the same step timed on a
+bounded sample of the same workload on this host, one core).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
+           "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill"]
+
+
+def words(D, S, G):
+    """Algorithmic words per cell of each reference subroutine: the distinct
+    per-cell fields read and written once (fieldn, tilde, iter setup, one
+    iter sweep, ygas, htilde+ftem)."""
+    P = S + 1
+    A = P * (P + 1) // 2
+    return {
+        "F": (5 + 2 * S + D * P + 2 * G) + (6 + 2 * S + G + D * P),
+        "T": (2 * P + 2 * D * P + 3) + (D * P + D * A),
+        "I0": (D * P + D * A + 4 + S) + (D * P + D + 2),
+        "Is": (5 + 2 * S + G + 3 * D * P + D * A + 2) + (5 + S + 2 * D * P),
+        "Y": (1 + 2 * G + D) + (2 * G),
+        "H": (13 + 5 * S + D * P + G) + (3 + 3 * S + G),
+    }
+
+
+# kernels of each reference subroutine
+GROUPS = {"F": ["fieldn"], "T": ["momflux", "tilde"], "I0": ["predict", "gasflux"],
+          "Is": ["snapshot", "sweep"], "Y": ["ygas_flux", "ygas_update"],
+          "H": ["htilde_flux", "htilde_update", "ftem"]}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_case(nx, nz):
+    from pdac_amd.cases import eruption_column
+    return eruption_column(nx=nx, nz=nz, nsteps=1)
+
+
+def profile(g, ids):
+    import ctypes as ct
+    out = {}
+    for name in KERNELS:
+        n, ms = ct.c_int64(), ct.c_double()
+        rc = g.lib.pdac_profile_read(g.ctx, ids[name], ct.byref(n), ct.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"pdac_profile_read: {g.last_error()}")
+        out[name] = (n.value, ms.value)
+    return out
+
+
+def cpu_sample(case, nsteps):
+    """The CPU restatement (oracle) on the same case, from the same state."""
+    from oracle.binding import cpu_solver
+    from pdac_amd import abi
+    c = cpu_solver(case, iter_order=abi.ORDER_REDBLACK)
+    t0 = time.perf_counter()
+    nits = []
+    for _ in range(nsteps):
+        c.step(1)
+        nits.append(c.nit_last)
+    t = time.perf_counter() - t0
+    c.close()
+    return t, nits
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=8,
+                    help="steps of the CPU sample (0: skip the CPU baseline)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the whole domain lives on one MI355X "
+              f"(--gpus {args.gpus} requested).", file=sys.stderr)
+        sys.exit(2)
+    import ctypes as ct
+
+    from pdac_amd.gpu import gpu_solver
+
+    case = make_case(args.nx, args.nz)
+    n = case.ntot
+    D, S, G = (3 if case.three_d else 2), case.nsolid, len(case.gas_type)
+    log(f"case {args.nx}x{args.nz} ({n} cells), dt {case.dt:.3e} s")
+    g = gpu_solver(case, device=0)
+    ids = {name: i for i, name in enumerate(KERNELS)}
+    t0 = time.perf_counter()
+    if args.warmup:
+        g.step(args.warmup)
+    g.fn["sync"] = g.lib.pdac_sync
+    g.lib.pdac_sync(g.ctx)
+    log(f"warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s, nit {g.nit_last}")
+    g.lib.pdac_profile_enable(g.ctx, 1)
+    g.lib.pdac_profile_reset(g.ctx)
+    nit_before = g.nit_total
+    g.lib.pdac_sync(g.ctx)
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.lib.pdac_sync(g.ctx)
+    elapsed = time.perf_counter() - t0
+    sweeps = g.nit_total - nit_before
+    prof = profile(g, ids)
+    g.lib.pdac_profile_enable(g.ctx, 0)
+    log(f"timed {args.steps} steps: {elapsed:.3f} s, {sweeps} sweeps")
+    value = n * args.steps / elapsed / 1e6
+    # dominant kernel and its roofline
+    w = words(D, S, G)
+    dom = max(KERNELS, key=lambda k: prof[k][1])
+    launches, ms = prof[dom]
+    per_launch_s = ms / 1e3 / max(launches, 1)
+    if dom == "sweep":
+        # one launch visits one colour, i.e. half of the cells
+        bytes_launch = 8.0 * w["Is"] * n / 2.0
+        unit_note = "cells visited per launch = ntot/2 (one colour)"
+    else:
+        grp = next(k for k, v in GROUPS.items() if dom in v)
+        share = len(GROUPS[grp])
+        bytes_launch = 8.0 * w[grp] * n / share
+        unit_note = f"ntot cells, 1/{share} of subroutine group {grp}"
+    achieved = bytes_launch / per_launch_s / 1e9
+    # whole-step algorithmic bytes
+    steps_per_s = args.steps / elapsed
+    b_step = 8.0 * (w["F"] + w["T"] + w["I0"] + w["Y"] + w["H"]) + 8.0 * w["Is"] * sweeps / args.steps
+    step_gbs = b_step * n * steps_per_s / 1e9
+    kern = {k: {"launches": prof[k][0], "ms_total": round(prof[k][1], 3)} for k in KERNELS if prof[k][0]}
+    out = {
+        "metric": "Mcell-steps/s",
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D cylindrical eruption column {args.nx}x{args.nz}, "
+                               f"2 particle classes, semi-implicit FP64 (plain form)",
+                   "ntot": n, "nsolid": S, "ngas": G, "rungekut": case.rungekut,
+                   "dt": case.dt, "sweeps_per_step": round(sweeps / args.steps, 3),
+                   "iter_order": "red-black"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_launch, "mean_launch_us": round(per_launch_s * 1e6, 2),
+                     "launches": launches, "units": unit_note},
+        "step_alg_gbs": round(step_gbs, 2),
+        "step_alg_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "kernels": kern,
+    }
+    g.close()
+    if args.cpu_steps > 0:
+        t, nits = cpu_sample(make_case(args.nx, args.nz), args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": round(n * args.cpu_steps / t / 1e6, 5), "unit": "Mcell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{args.cpu_steps} steps of the same {args.nx}x{args.nz} case from its "
+                      f"initial state (sweeps {nits}), {t:.1f} s on 1 host core"}
+        log(f"cpu sample: {t:.1f} s for {args.cpu_steps} steps")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

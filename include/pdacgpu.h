@@ -158,6 +158,19 @@ const char *pdac_last_error(void);
  * out receives ntot*(nsolid+1) values, phase-major. */
 int pdac_get_momflux(pdac_ctx *ctx, int comp, int face, double *out);
 
+/* Diagnostic: per-kernel timing with HIP events recorded on the context's
+ * stream around each launch (no counterpart in the reference, whose phase
+ * timers are prog.f:430-451).  Sweep launches that found the iteration
+ * already converged are not counted.  pdac_profile_read synchronises. */
+enum pdac_kernel {
+  PDAC_K_BOUNDARY = 0, PDAC_K_FIELDN, PDAC_K_MOMFLUX, PDAC_K_TILDE, PDAC_K_PREDICT,
+  PDAC_K_GASFLUX, PDAC_K_SNAPSHOT, PDAC_K_SWEEP, PDAC_K_YGAS_FLUX, PDAC_K_YGAS_UPDATE,
+  PDAC_K_HTILDE_FLUX, PDAC_K_HTILDE_UPDATE, PDAC_K_FTEM, PDAC_K_FILL, PDAC_K_COUNT
+};
+int pdac_profile_enable(pdac_ctx *ctx, int enable);
+int pdac_profile_reset(pdac_ctx *ctx);
+int pdac_profile_read(pdac_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif
