@@ -98,11 +98,11 @@ def cpu_sample(case, nsteps):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--nz", type=int, default=1024)
-    ap.add_argument("--cpu-steps", type=int, default=8,
+    ap.add_argument("--cpu-steps", type=int, default=16,
                     help="steps of the CPU sample (0: skip the CPU baseline)")
     args = ap.parse_args()
     if args.gpus != 1:
