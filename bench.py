@@ -33,8 +33,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# order of enum pdac_kernel (include/pdacgpu.h)
 KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
-           "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill"]
+           "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill",
+           "sweep_check"]
 
 
 def words(D, S, G):
@@ -55,7 +57,7 @@ def words(D, S, G):
 
 # kernels of each reference subroutine
 GROUPS = {"F": ["fieldn"], "T": ["momflux", "tilde"], "I0": ["predict", "gasflux"],
-          "Is": ["snapshot", "sweep"], "Y": ["ygas_flux", "ygas_update"],
+          "Is": ["snapshot", "sweep_check", "sweep"], "Y": ["ygas_flux", "ygas_update"],
           "H": ["htilde_flux", "htilde_update", "ftem"]}
 
 
@@ -135,6 +137,10 @@ def main():
     elapsed = time.perf_counter() - t0
     sweeps = g.nit_total - nit_before
     prof = profile(g, ids)
+    visits = ct.c_int64()
+    if g.lib.pdac_profile_visits(g.ctx, ct.byref(visits)) != 0:
+        raise RuntimeError(g.last_error())
+    visits = visits.value
     g.lib.pdac_profile_enable(g.ctx, 0)
     log(f"timed {args.steps} steps: {elapsed:.3f} s, {sweeps} sweeps")
     value = n * args.steps / elapsed / 1e6
@@ -144,9 +150,11 @@ def main():
     launches, ms = prof[dom]
     per_launch_s = ms / 1e3 / max(launches, 1)
     if dom == "sweep":
-        # one launch visits one colour, i.e. half of the cells
-        bytes_launch = 8.0 * w["Is"] * n / 2.0
-        unit_note = "cells visited per launch = ntot/2 (one colour)"
+        # one launch runs the in-cell iteration of the cells of one colour that
+        # were not converged at visit start (the work list)
+        bytes_launch = 8.0 * w["Is"] * visits / max(launches, 1)
+        unit_note = (f"cells per launch = mean work-list length {visits / max(launches, 1):.0f} "
+                     f"(cells of one colour not converged at visit start), {w['Is']} words each")
     else:
         grp = next(k for k, v in GROUPS.items() if dom in v)
         share = len(GROUPS[grp])
@@ -181,6 +189,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": bytes_launch, "mean_launch_us": round(per_launch_s * 1e6, 2),
                      "launches": launches, "units": unit_note},
+        "sweep_visits_per_step": round(visits / args.steps, 1),
         "step_alg_gbs": round(step_gbs, 2),
         "step_alg_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "kernels": kern,

@@ -165,11 +165,15 @@ int pdac_get_momflux(pdac_ctx *ctx, int comp, int face, double *out);
 enum pdac_kernel {
   PDAC_K_BOUNDARY = 0, PDAC_K_FIELDN, PDAC_K_MOMFLUX, PDAC_K_TILDE, PDAC_K_PREDICT,
   PDAC_K_GASFLUX, PDAC_K_SNAPSHOT, PDAC_K_SWEEP, PDAC_K_YGAS_FLUX, PDAC_K_YGAS_UPDATE,
-  PDAC_K_HTILDE_FLUX, PDAC_K_HTILDE_UPDATE, PDAC_K_FTEM, PDAC_K_FILL, PDAC_K_COUNT
+  PDAC_K_HTILDE_FLUX, PDAC_K_HTILDE_UPDATE, PDAC_K_FTEM, PDAC_K_FILL, PDAC_K_SWEEP_CHECK,
+  PDAC_K_COUNT
 };
 int pdac_profile_enable(pdac_ctx *ctx, int enable);
 int pdac_profile_reset(pdac_ctx *ctx);
 int pdac_profile_read(pdac_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
+/* cells visited by PDAC_K_SWEEP launches (not converged at visit start)
+ * since the last pdac_profile_reset */
+int pdac_profile_visits(pdac_ctx *ctx, int64_t *visits);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,8 @@ struct Fields {
   double *appu_old[3];
   double *snap_rgp, *snap_rlk;
   int32_t *ctr;                        // device counters (iter, ftem)
+  unsigned long long *visits;          // work-list entries over all sweeps
+  int32_t *wl;                         // work list of one colour phase
 };
 
 // --------------------------------------------------------------------

@@ -38,7 +38,16 @@ int set_error(const char *where, hipError_t e) {
 
 constexpr int kBlock = 256;
 
-inline unsigned nblocks(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+constexpr unsigned kXcd = 8;  // MI355X: 8 accelerator dies, each with its own L2
+
+// Work-groups are dealt to the XCDs round-robin.  Grids are rounded up to a
+// multiple of 8 blocks and block b works on the contiguous range
+// (b % 8) * (nblocks / 8) + b / 8, so each XCD sweeps one contiguous slab of
+// the mesh and a cell's k+-1 neighbours sit in the same L2.
+inline unsigned nblocks(long n) {
+  unsigned b = (unsigned)((n + kBlock - 1) / kBlock);
+  return (b + kXcd - 1) / kXcd * kXcd;
+}
 
 // decode a linear cell index into 1-based (i,j,k)
 __device__ __forceinline__ void decode(const Grid &g, long c, int &i, int &j, int &k) {
@@ -49,14 +58,28 @@ __device__ __forceinline__ void decode(const Grid &g, long c, int &i, int &j, in
   i = (int)(r - (long)(j - 1) * g.nx) + 1;
 }
 
-// wavefront-level reduction of a per-lane count, one atomic per wavefront
-__device__ __forceinline__ void wave_count_add(int32_t *ctr, bool pred) {
-  unsigned long long m = __ballot(pred);
-  if (m && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1))
-    atomicAdd(ctr, (int32_t)__popcll(m));
+__device__ __forceinline__ long cell_id() {
+  const unsigned per = gridDim.x / kXcd;
+  const unsigned b = blockIdx.x;
+  return (long)((b % kXcd) * per + b / kXcd) * blockDim.x + threadIdx.x;
 }
 
-enum Ctr { C_DONE = 0, C_NOTCONV = 1, C_NIT = 2, C_IERR = 3, C_EOS = 4, C_N = 8 };
+// Append 'val' to a device work list from every lane where 'pred' holds: one
+// atomic per wavefront, lanes placed in lane order.  Every lane of the
+// wavefront must call it.
+__device__ __forceinline__ void wl_append(int32_t *cnt, int32_t *list, bool pred, int32_t val) {
+  const unsigned long long m = __ballot(pred);
+  if (!m) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(cnt, (int32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = val;
+}
+
+// device counters: sweep state of iter, work-list lengths per colour, EOS failures
+enum Ctr { C_DONE = 0, C_NIT = 2, C_EOS = 4, C_WL0 = 5, C_WL1 = 6, C_N = 8 };
 
 }  // namespace
 
@@ -149,7 +172,7 @@ __device__ __forceinline__ double extrapolate(double f2, double f1, double d12, 
 
 template <int D, int NS>
 __global__ void __launch_bounds__(kBlock) k_boundary(Grid g, Consts K, Fields F) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = cell_id();
   if (c >= g.n) return;
   const int32_t *fl = g.flag;
   int f = fl[c];
@@ -358,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) k_boundary(Grid g, Consts K, Fields F)
 // fieldn, tilde.f:125-274: step-start copies, face momenta, viscosity
 template <int D, int NS>
 __global__ void __launch_bounds__(kBlock) k_fieldn(Grid g, Consts K, Fields F) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = cell_id();
   if (c >= g.n) return;
   long n = g.n;
   const double *rgp = F.f[PDAC_RGP], *rlk = F.f[PDAC_RLK];
@@ -414,12 +437,12 @@ __global__ void __launch_bounds__(kBlock) k_fieldn(Grid g, Consts K, Fields F) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_fill(double *a, long n, double v) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = cell_id();
   if (c < n) a[c] = v;
 }
 
 __global__ void __launch_bounds__(kBlock) k_copy(double *dst, const double *src, long n) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = cell_id();
   if (c < n) dst[c] = src[c];
 }
 
@@ -430,7 +453,7 @@ __global__ void __launch_bounds__(kBlock) k_copy(double *dst, const double *src,
 // not computed (then no other cell writes it).
 template <int D, int NS>
 __global__ void __launch_bounds__(kBlock) k_momflux(Grid g, Consts K, Fields F) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = cell_id();
   if (c >= g.n) return;
   int fc = g.flag[c];
   if (!computed(fc)) return;
@@ -530,7 +553,7 @@ __device__ __forceinline__ void inter_d(const Consts &K, const double *kpgv, dou
 // east/north/top neighbours before it reaches them).
 template <int D, int NS>
 __global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, double dt, int old_zero) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long c = cell_id();
   if (c >= g.n) return;
   if (!computed(g.flag[c])) return;
   constexpr int NP = NS + 1;
