@@ -62,6 +62,8 @@ struct Fields {
   int32_t *ctr;                        // device counters (iter, ftem)
   unsigned long long *visits;          // work-list entries over all sweeps
   int32_t *wl;                         // work list of one colour phase
+  uint8_t *dirty;                      // cell must be checked again (iter)
+  int32_t *stamp;                      // colour phase whose start snap_* holds
 };
 
 // --------------------------------------------------------------------
