@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # order of enum pdac_kernel (include/pdacgpu.h)
 KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
            "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill",
-           "sweep_check"]
+           "sweep_check", "iterate"]
 
 
 def words(D, S, G):
@@ -57,7 +57,7 @@ def words(D, S, G):
 
 # kernels of each reference subroutine
 GROUPS = {"F": ["fieldn"], "T": ["momflux", "tilde"], "I0": ["predict", "gasflux"],
-          "Is": ["snapshot", "sweep_check", "sweep"], "Y": ["ygas_flux", "ygas_update"],
+          "Is": ["snapshot", "sweep_check", "sweep", "iterate"], "Y": ["ygas_flux", "ygas_update"],
           "H": ["htilde_flux", "htilde_update", "ftem"]}
 
 
@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--nz", type=int, default=1024)
+    ap.add_argument("--kernel-timing", choices=["all", "dominant", "none"], default="dominant",
+                    help="HIP events around every kernel, around the iteration's work-list "
+                         "kernel only (the dominant one), or none")
     ap.add_argument("--cpu-steps", type=int, default=16,
                     help="steps of the CPU sample (0: skip the CPU baseline)")
     args = ap.parse_args()
@@ -127,7 +130,10 @@ def main():
     g.fn["sync"] = g.lib.pdac_sync
     g.lib.pdac_sync(g.ctx)
     log(f"warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s, nit {g.nit_last}")
-    g.lib.pdac_profile_enable(g.ctx, 1)
+    if args.kernel_timing != "none":
+        mask = 0xFFFFFFFF if args.kernel_timing == "all" else (1 << ids["sweep"]) | (1 << ids["iterate"])
+        g.lib.pdac_profile_select(g.ctx, mask)
+        g.lib.pdac_profile_enable(g.ctx, 1)
     g.lib.pdac_profile_reset(g.ctx)
     nit_before = g.nit_total
     g.lib.pdac_sync(g.ctx)
@@ -148,13 +154,22 @@ def main():
     w = words(D, S, G)
     dom = max(KERNELS, key=lambda k: prof[k][1])
     launches, ms = prof[dom]
-    per_launch_s = ms / 1e3 / max(launches, 1)
+    if launches == 0:
+        dom, ms = "none", 0.0
+    per_launch_s = ms / 1e3 / max(launches, 1) if launches else float("nan")
     if dom == "sweep":
         # one launch runs the in-cell iteration of the cells of one colour that
         # were not converged at visit start (the work list)
         bytes_launch = 8.0 * w["Is"] * visits / max(launches, 1)
         unit_note = (f"cells per launch = mean work-list length {visits / max(launches, 1):.0f} "
                      f"(cells of one colour not converged at visit start), {w['Is']} words each")
+    elif dom == "iterate":
+        # one launch runs every sweep of one iter call
+        bytes_launch = 8.0 * w["Is"] * n * sweeps / max(launches, 1)
+        unit_note = (f"ntot cells x sweeps per launch ({sweeps / max(launches, 1):.2f}), "
+                     f"{w['Is']} words per cell and sweep")
+    elif dom == "none":
+        bytes_launch, unit_note = 0.0, "kernel timing off"
     else:
         grp = next(k for k, v in GROUPS.items() if dom in v)
         share = len(GROUPS[grp])

@@ -166,9 +166,11 @@ enum pdac_kernel {
   PDAC_K_BOUNDARY = 0, PDAC_K_FIELDN, PDAC_K_MOMFLUX, PDAC_K_TILDE, PDAC_K_PREDICT,
   PDAC_K_GASFLUX, PDAC_K_SNAPSHOT, PDAC_K_SWEEP, PDAC_K_YGAS_FLUX, PDAC_K_YGAS_UPDATE,
   PDAC_K_HTILDE_FLUX, PDAC_K_HTILDE_UPDATE, PDAC_K_FTEM, PDAC_K_FILL, PDAC_K_SWEEP_CHECK,
-  PDAC_K_COUNT
+  PDAC_K_ITERATE, PDAC_K_COUNT
 };
 int pdac_profile_enable(pdac_ctx *ctx, int enable);
+/* bit k of mask: time kernel k (default: all) */
+int pdac_profile_select(pdac_ctx *ctx, uint32_t mask);
 int pdac_profile_reset(pdac_ctx *ctx);
 int pdac_profile_read(pdac_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
 /* cells visited by PDAC_K_SWEEP launches (not converged at visit start)

@@ -22,7 +22,8 @@ EXPORTS = [
     "pdac_get", "pdac_field_ncomp", "pdac_cnvert", "pdac_boundary", "pdac_fieldn",
     "pdac_tilde", "pdac_iter", "pdac_ygas", "pdac_htilde", "pdac_ftem",
     "pdac_begin_step", "pdac_step", "pdac_sync", "pdac_last_error", "pdac_get_momflux",
-    "pdac_profile_enable", "pdac_profile_reset", "pdac_profile_read", "pdac_profile_visits",
+    "pdac_profile_enable", "pdac_profile_select", "pdac_profile_reset", "pdac_profile_read",
+    "pdac_profile_visits",
 ]
 
 
@@ -39,6 +40,7 @@ def lib() -> ct.CDLL:
         L.pdac_get_momflux.restype = ct.c_int
         L.pdac_profile_enable.argtypes = [ct.c_void_p, ct.c_int]
         L.pdac_profile_reset.argtypes = [ct.c_void_p]
+        L.pdac_profile_select.argtypes = [ct.c_void_p, ct.c_uint32]
         L.pdac_profile_read.argtypes = [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64),
                                         ct.POINTER(ct.c_double)]
         L.pdac_profile_visits.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int64)]

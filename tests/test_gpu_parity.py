@@ -29,9 +29,17 @@ ALL = sorted(CASES)
 CONVERGING = [c for c in ALL if c != "mixed_2d_limiters"]
 
 
-def gpu(case):
+def gpu(case, mode="persistent"):
+    """GPU solver; mode selects how the red-black sweeps are launched: all in
+    one persistent kernel, or two launches per colour phase."""
+    import os
+
     from pdac_amd.gpu import gpu_solver
-    return gpu_solver(case, device=0)
+    os.environ["PDAC_ITER_MODE"] = mode
+    try:
+        return gpu_solver(case, device=0)
+    finally:
+        os.environ.pop("PDAC_ITER_MODE", None)
 
 
 def rel(a, b):
@@ -65,11 +73,12 @@ STAGES = [
 ]
 
 
+@pytest.mark.parametrize("mode", ["persistent", "launches"])
 @pytest.mark.parametrize("name", ALL)
-def test_stagewise(name):
-    print(f"[stagewise] {name}", flush=True)
+def test_stagewise(name, mode):
+    print(f"[stagewise] {name} {mode}", flush=True)
     case, _, _ = load(name)
-    g, c = gpu(case), cpu_solver(case, iter_order=abi.ORDER_REDBLACK)
+    g, c = gpu(case, mode), cpu_solver(case, iter_order=abi.ORDER_REDBLACK)
     sync(g, c)
     dt = case.dt
     for step in range(min(case.nsteps, 2)):
@@ -112,11 +121,12 @@ def test_momentum_fluxes(name):
             assert rel(got, want) <= 1e-14, (comp, face, rel(got, want))
 
 
+@pytest.mark.parametrize("mode", ["persistent", "launches"])
 @pytest.mark.parametrize("name", ALL)
-def test_steps_match_checker(name):
-    print(f"[steps] {name}", flush=True)
+def test_steps_match_checker(name, mode):
+    print(f"[steps] {name} {mode}", flush=True)
     case, _, _ = load(name)
-    g, c = gpu(case), cpu_solver(case, iter_order=abi.ORDER_REDBLACK)
+    g, c = gpu(case, mode), cpu_solver(case, iter_order=abi.ORDER_REDBLACK)
     ng, nc = [], []
     for _ in range(case.nsteps):
         ng.append(g.step(1))
