@@ -28,6 +28,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -97,39 +99,22 @@ def cpu_sample(case, nsteps):
     return t, nits
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=512)
-    ap.add_argument("--nz", type=int, default=1024)
-    ap.add_argument("--kernel-timing", choices=["all", "dominant", "none"], default="dominant",
-                    help="HIP events around every kernel, around the iteration's work-list "
-                         "kernel only (the dominant one), or none")
-    ap.add_argument("--cpu-steps", type=int, default=16,
-                    help="steps of the CPU sample (0: skip the CPU baseline)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: the whole domain lives on one MI355X "
-              f"(--gpus {args.gpus} requested).", file=sys.stderr)
-        sys.exit(2)
+def run_form(args, precision, ids):
+    """Warm-up, the timed region and the event-timed region of one precision
+    form on a fresh copy of the workload; returns timings and the final p and
+    particle volume fractions."""
     import ctypes as ct
 
     from pdac_amd.gpu import gpu_solver
-
     case = make_case(args.nx, args.nz)
     n = case.ntot
-    D, S, G = (3 if case.three_d else 2), case.nsolid, len(case.gas_type)
-    log(f"case {args.nx}x{args.nz} ({n} cells), dt {case.dt:.3e} s")
-    g = gpu_solver(case, device=0)
-    ids = {name: i for i, name in enumerate(KERNELS)}
+    g = gpu_solver(case, device=0, precision=precision)
     t0 = time.perf_counter()
     if args.warmup:
         g.step(args.warmup)
-    g.fn["sync"] = g.lib.pdac_sync
     g.lib.pdac_sync(g.ctx)
-    log(f"warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s, nit {g.nit_last}")
+    tag = "fp32-flux" if precision else "fp64"
+    log(f"[{tag}] warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s, nit {g.nit_last}")
     # timed region: K steps, nothing recorded but the stream synchronisations
     nit_before = g.nit_total
     g.lib.pdac_sync(g.ctx)
@@ -138,8 +123,7 @@ def main():
     g.lib.pdac_sync(g.ctx)
     elapsed = time.perf_counter() - t0
     sweeps = g.nit_total - nit_before
-    log(f"timed {args.steps} steps: {elapsed:.3f} s, {sweeps} sweeps")
-    value = n * args.steps / elapsed / 1e6
+    log(f"[{tag}] timed {args.steps} steps: {elapsed:.3f} s, {sweeps} sweeps")
     # second region of the same length with HIP events around the kernels
     # (recording events costs host time per launch, so it is kept out of
     # the timed region above)
@@ -158,9 +142,55 @@ def main():
     visits = ct.c_int64()
     if g.lib.pdac_profile_visits(g.ctx, ct.byref(visits)) != 0:
         raise RuntimeError(g.last_error())
-    visits = visits.value
     g.lib.pdac_profile_enable(g.ctx, 0)
-    log(f"event-timed {args.steps} steps: {elapsed_prof:.3f} s, {sweeps_prof} sweeps")
+    log(f"[{tag}] event-timed {args.steps} steps: {elapsed_prof:.3f} s, {sweeps_prof} sweeps")
+    rho = np.asarray(case.density[:case.nsolid], dtype=np.float64).reshape(-1, 1)
+    final = {"p": g.get("p"), "eps_s": g.get("rlk").reshape(case.nsolid, -1) / rho}
+    g.close()
+    return {"case": case, "n": n, "elapsed": elapsed, "sweeps": sweeps, "elapsed_prof": elapsed_prof,
+            "sweeps_prof": sweeps_prof, "prof": prof, "visits": visits.value, "final": final}
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--nz", type=int, default=1024)
+    ap.add_argument("--kernel-timing", choices=["all", "dominant", "none"], default="dominant",
+                    help="HIP events around every kernel, around the iteration's work-list "
+                         "kernel only (the dominant one), or none")
+    ap.add_argument("--forms", choices=["both", "fp64", "fp32flux"], default="both",
+                    help="precision forms to run; with both, value is the FP32-flux form's")
+    ap.add_argument("--cpu-steps", type=int, default=16,
+                    help="steps of the CPU sample (0: skip the CPU baseline)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the whole domain lives on one MI355X "
+              f"(--gpus {args.gpus} requested).", file=sys.stderr)
+        sys.exit(2)
+    from pdac_amd import abi
+
+    ids = {name: i for i, name in enumerate(KERNELS)}
+    runs = {}
+    if args.forms in ("both", "fp64"):
+        runs["fp64"] = run_form(args, abi.PREC_FP64, ids)
+    if args.forms in ("both", "fp32flux"):
+        runs["fp32flux"] = run_form(args, abi.PREC_FP32FLUX, ids)
+    main_form = "fp32flux" if "fp32flux" in runs else "fp64"
+    r = runs[main_form]
+    case, n = r["case"], r["n"]
+    D, S, G = (3 if case.three_d else 2), case.nsolid, len(case.gas_type)
+    elapsed, sweeps, prof, visits = r["elapsed"], r["sweeps"], r["prof"], r["visits"]
+    sweeps_prof = r["sweeps_prof"]
+    value = n * args.steps / elapsed / 1e6
     # dominant kernel and its roofline
     w = words(D, S, G)
     dom = max(KERNELS, key=lambda k: prof[k][1])
@@ -203,10 +233,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if main_form == "fp64" else "f64 with f32 flux arithmetic",
         "data": "synthetic",
         "config": {"workload": f"2D cylindrical eruption column {args.nx}x{args.nz}, "
-                               f"2 particle classes, semi-implicit FP64 (plain form)",
+                               f"2 particle classes, semi-implicit",
+                   "form": "FP32-flux" if main_form == "fp32flux" else "FP64 plain",
                    "ntot": n, "nsolid": S, "ngas": G, "rungekut": case.rungekut,
                    "dt": case.dt, "sweeps_per_step": round(sweeps / args.steps, 3),
                    "iter_order": "red-black"},
@@ -216,14 +247,21 @@ def main():
                      "bytes_per_launch": bytes_launch, "mean_launch_us": round(per_launch_s * 1e6, 2),
                      "launches": launches, "units": unit_note},
         "sweep_visits_per_step": round(visits / args.steps, 1),
-        "event_region": {"steps": args.steps, "seconds": round(elapsed_prof, 4),
+        "event_region": {"steps": args.steps, "seconds": round(r["elapsed_prof"], 4),
                          "sweeps_per_step": round(sweeps_prof / args.steps, 3),
                          "timing": args.kernel_timing},
         "step_alg_gbs": round(step_gbs, 2),
         "step_alg_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "kernels": kern,
     }
-    g.close()
+    if len(runs) == 2:
+        a, b = runs["fp32flux"], runs["fp64"]
+        out["fp64_value"] = round(n * args.steps / b["elapsed"] / 1e6, 3)
+        out["fp32flux_speedup"] = round(b["elapsed"] / a["elapsed"], 4)
+        out["fp32flux_rel_l2"] = {
+            "after_steps": args.warmup + 2 * args.steps,
+            "p": rel_l2(a["final"]["p"], b["final"]["p"]),
+            "eps_s": [rel_l2(a["final"]["eps_s"][s_], b["final"]["eps_s"][s_]) for s_ in range(S)]}
     if args.cpu_steps > 0:
         t, nits = cpu_sample(make_case(args.nx, args.nz), args.cpu_steps)
         out["cpu_baseline"] = {

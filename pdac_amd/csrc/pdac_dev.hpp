@@ -163,45 +163,49 @@ __host__ __device__ __forceinline__ long refl_diag2(const int32_t *fl, long base
 }
 
 // ----------------------------------------------------------------------
-// limiters, limiters.f:14-56
-__device__ __forceinline__ double limiter(int t, double beta, double erre) {
-  double l, m, bu;
+// limiters, limiters.f:14-56 (T = double for the plain form, float for the
+// flux arithmetic of the FP32-flux form)
+template <typename T>
+__device__ __forceinline__ T limiter(int t, T beta, T erre) {
+  T l, m, bu;
   switch (t) {
-  case 1: l = 2.0 * erre / (1.0 + erre); return l > 0.0 ? l : 0.0;
-  case 2: m = erre < 1.0 ? erre : 1.0; return m > 0.0 ? m : 0.0;
+  case 1: l = T(2) * erre / (T(1) + erre); return l > T(0) ? l : T(0);
+  case 2: m = erre < T(1) ? erre : T(1); return m > T(0) ? m : T(0);
   case 3: {
-    double a = 2.0 * erre < 1.0 ? 2.0 * erre : 1.0;
-    double b = erre < 2.0 ? erre : 2.0;
-    double r = a > b ? a : b;
-    return r > 0.0 ? r : 0.0;
+    T a = T(2) * erre < T(1) ? T(2) * erre : T(1);
+    T b = erre < T(2) ? erre : T(2);
+    T r = a > b ? a : b;
+    return r > T(0) ? r : T(0);
   }
   case 4:
-    bu = (1.0 - beta) + beta * erre;
-    m = 2.0 * erre;
+    bu = (T(1) - beta) + beta * erre;
+    m = T(2) * erre;
     if (bu < m) m = bu;
-    if (2.0 < m) m = 2.0;
-    return m > 0.0 ? m : 0.0;
-  case 0: return (1.0 - beta) + beta * erre;
-  default: return 0.0;
+    if (T(2) < m) m = T(2);
+    return m > T(0) ? m : T(0);
+  case 0: return (T(1) - beta) + beta * erre;
+  default: return T(0);
   }
 }
 
 // one MUSCL face increment: ratio of upwind to central gradient, limited,
 // times the half-cell increment (flux_m.f:165-184 and its analogues)
-__device__ __forceinline__ double muscl_incr(int lim_type, double beta, bool use_lim,
-                                             double gradc, double gradup, double graddn,
-                                             double cs, double incr_p, double incr_m) {
-  double erre = 0.0, incr;
-  if (cs >= 0.0) { if (gradc != 0.0) erre = gradup / gradc; incr = incr_p; }
-  else { if (gradc != 0.0) erre = graddn / gradc; incr = incr_m; }
-  double lim = use_lim ? limiter(lim_type, beta, erre) : 0.0;
+template <typename T>
+__device__ __forceinline__ T muscl_incr(int lim_type, T beta, bool use_lim, T gradc, T gradup,
+                                        T graddn, T cs, T incr_p, T incr_m) {
+  T erre = T(0), incr;
+  if (cs >= T(0)) { if (gradc != T(0)) erre = gradup / gradc; incr = incr_p; }
+  else { if (gradc != T(0)) erre = graddn / gradc; incr = incr_m; }
+  T lim = use_lim ? limiter<T>(lim_type, beta, erre) : T(0);
   return lim * gradc * incr;
 }
 
 // donor-cell flux of a face: 0.5(cs-|cs|)*down + 0.5(cs+|cs|)*up with the
 // reference's left-to-right products
-__device__ __forceinline__ double upw(double cs) { return 0.5 * (cs - fabs(cs)); }
-__device__ __forceinline__ double dnw(double cs) { return 0.5 * (cs + fabs(cs)); }
+template <typename T>
+__device__ __forceinline__ T upw(T cs) { return T(0.5) * (cs - fabs(cs)); }
+template <typename T>
+__device__ __forceinline__ T dnw(T cs) { return T(0.5) * (cs + fabs(cs)); }
 
 // ----------------------------------------------------------------------
 // thermodynamics (eosg.f, hcapgs.f)

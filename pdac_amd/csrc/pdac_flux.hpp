@@ -8,11 +8,19 @@
 
 namespace pdac {
 
+// a grid array (dx, 1/dx, r, ...) read in the arithmetic type T
+template <typename T>
+struct GA {
+  const double *p;
+  __device__ __forceinline__ T operator[](int i) const { return T(p[i]); }
+};
+
 // values of one field on the stencil of a cell (subscr.f:63-210 names)
+template <typename T>
 struct Stc {
-  double c, e, w, n, s, t, b, ee, ww, nn, ss, tt, bb;
-  double en, es, et, eb, nt, wn, nb, wt, st;
-  double eee, nnn, ttt, een, eet, enn, nnt, ett, ntt;
+  T c, e, w, n, s, t, b, ee, ww, nn, ss, tt, bb;
+  T en, es, et, eb, nt, wn, nb, wt, st;
+  T eee, nnn, ttt, een, eet, enn, nnt, ett, ntt;
 };
 
 // Full stencil indices of a computed cell used by the momentum fluxes:
@@ -91,39 +99,39 @@ __device__ __forceinline__ void full_idx(const Grid &g, int i, int j, int k, Ful
 }
 
 // reflected full stencil of a density (nb, subscr.f:2323-2383)
-template <int D>
-__device__ __forceinline__ void gather_nb(const FullIdx<D> &x, const double *a, Stc &o) {
-  o.c = a[x.c]; o.e = a[x.Re]; o.w = a[x.Rw]; o.t = a[x.Rt]; o.b = a[x.Rb];
-  o.ee = a[x.Ree]; o.tt = a[x.Rtt]; o.et = a[x.Ret]; o.eb = a[x.Reb]; o.wt = a[x.Rwt];
-  o.eee = a[x.Reee]; o.ttt = a[x.Rttt]; o.eet = a[x.Reet]; o.ett = a[x.Rett];
+template <int D, typename T>
+__device__ __forceinline__ void gather_nb(const FullIdx<D> &x, const double *a, Stc<T> &o) {
+  o.c = T(a[x.c]); o.e = T(a[x.Re]); o.w = T(a[x.Rw]); o.t = T(a[x.Rt]); o.b = T(a[x.Rb]);
+  o.ee = T(a[x.Ree]); o.tt = T(a[x.Rtt]); o.et = T(a[x.Ret]); o.eb = T(a[x.Reb]); o.wt = T(a[x.Rwt]);
+  o.eee = T(a[x.Reee]); o.ttt = T(a[x.Rttt]); o.eet = T(a[x.Reet]); o.ett = T(a[x.Rett]);
   if (D == 3) {
-    o.n = a[x.Rn]; o.s = a[x.Rs]; o.nn = a[x.Rnn]; o.en = a[x.Ren]; o.es = a[x.Res];
-    o.nt = a[x.Rnt]; o.wn = a[x.Rwn]; o.nb = a[x.Rnb]; o.st = a[x.Rst];
-    o.nnn = a[x.Rnnn]; o.een = a[x.Reen]; o.enn = a[x.Renn]; o.nnt = a[x.Rnnt];
-    o.ntt = a[x.Rntt];
+    o.n = T(a[x.Rn]); o.s = T(a[x.Rs]); o.nn = T(a[x.Rnn]); o.en = T(a[x.Ren]); o.es = T(a[x.Res]);
+    o.nt = T(a[x.Rnt]); o.wn = T(a[x.Rwn]); o.nb = T(a[x.Rnb]); o.st = T(a[x.Rst]);
+    o.nnn = T(a[x.Rnnn]); o.een = T(a[x.Reen]); o.enn = T(a[x.Renn]); o.nnt = T(a[x.Rnnt]);
+    o.ntt = T(a[x.Rntt]);
   }
 }
 
 // raw stencil of a velocity (rnb_r, subscr.f:2386-2450), entries used by the
 // momentum fluxes
-template <int D>
-__device__ __forceinline__ void gather_rnb(const FullIdx<D> &x, const double *a, Stc &o) {
-  o.c = a[x.c]; o.e = a[x.e]; o.w = a[x.w]; o.t = a[x.t]; o.b = a[x.b];
-  o.ee = a[x.ee]; o.tt = a[x.tt]; o.wt = a[x.wt]; o.eb = a[x.eb];
+template <int D, typename T>
+__device__ __forceinline__ void gather_rnb(const FullIdx<D> &x, const double *a, Stc<T> &o) {
+  o.c = T(a[x.c]); o.e = T(a[x.e]); o.w = T(a[x.w]); o.t = T(a[x.t]); o.b = T(a[x.b]);
+  o.ee = T(a[x.ee]); o.tt = T(a[x.tt]); o.wt = T(a[x.wt]); o.eb = T(a[x.eb]);
   if (D == 3) {
-    o.n = a[x.n]; o.s = a[x.s]; o.nn = a[x.nn]; o.es = a[x.es]; o.st = a[x.st];
-    o.wn = a[x.wn]; o.nb = a[x.nb];
+    o.n = T(a[x.n]); o.s = T(a[x.s]); o.nn = T(a[x.nn]); o.es = T(a[x.es]); o.st = T(a[x.st]);
+    o.wn = T(a[x.wn]); o.nb = T(a[x.nb]);
   }
 }
 
 // ----------------------------------------------------------------------
 // staggered densities, interp.f:14-320 (entries the momentum fluxes read)
-template <int D>
-__device__ __forceinline__ void interp_x(const Grid &g, const Stc &f, Stc &o, int i) {
-  const double *dx = g.dx;
+template <int D, typename T>
+__device__ __forceinline__ void interp_x(const Grid &g, const Stc<T> &f, Stc<T> &o, int i) {
+  const GA<T> dx{g.dx};
   int ip2 = imin(i + 2, g.nx), ip3 = imin(i + 3, g.nx);
-  double indxm = 1.0 / (dx[i] + dx[i - 1]), indxp = 1.0 / (dx[i] + dx[i + 1]);
-  double indxpp = 1.0 / (dx[i + 1] + dx[ip2]), indxppp = 1.0 / (dx[ip2] + dx[ip3]);
+  T indxm = T(1) / (dx[i] + dx[i - 1]), indxp = T(1) / (dx[i] + dx[i + 1]);
+  T indxpp = T(1) / (dx[i + 1] + dx[ip2]), indxppp = T(1) / (dx[ip2] + dx[ip3]);
   o.c = (dx[i + 1] * f.c + dx[i] * f.e) * indxp;
   o.e = (dx[ip2] * f.e + dx[i + 1] * f.ee) * indxpp;
   o.t = (dx[i + 1] * f.t + dx[i] * f.et) * indxp;
@@ -138,11 +146,12 @@ __device__ __forceinline__ void interp_x(const Grid &g, const Stc &f, Stc &o, in
   }
 }
 
-__device__ __forceinline__ void interp_y(const Grid &g, const Stc &f, Stc &o, int j) {
-  const double *dy = g.dy;
+template <typename T>
+__device__ __forceinline__ void interp_y(const Grid &g, const Stc<T> &f, Stc<T> &o, int j) {
+  const GA<T> dy{g.dy};
   int jp2 = imin(j + 2, g.ny), jp3 = imin(j + 3, g.ny);
-  double indym = 1.0 / (dy[j] + dy[j - 1]), indyp = 1.0 / (dy[j] + dy[j + 1]);
-  double indypp = 1.0 / (dy[j + 1] + dy[jp2]), indyppp = 1.0 / (dy[jp2] + dy[jp3]);
+  T indym = T(1) / (dy[j] + dy[j - 1]), indyp = T(1) / (dy[j] + dy[j + 1]);
+  T indypp = T(1) / (dy[j + 1] + dy[jp2]), indyppp = T(1) / (dy[jp2] + dy[jp3]);
   o.c = (dy[j + 1] * f.c + dy[j] * f.n) * indyp;
   o.e = (dy[j + 1] * f.e + dy[j] * f.en) * indyp;
   o.n = (dy[jp2] * f.n + dy[j + 1] * f.nn) * indypp;
@@ -155,12 +164,12 @@ __device__ __forceinline__ void interp_y(const Grid &g, const Stc &f, Stc &o, in
   o.tt = (dy[j] * f.ntt + dy[j + 1] * f.tt) * indyp;
 }
 
-template <int D>
-__device__ __forceinline__ void interp_z(const Grid &g, const Stc &f, Stc &o, int k) {
-  const double *dz = g.dz;
+template <int D, typename T>
+__device__ __forceinline__ void interp_z(const Grid &g, const Stc<T> &f, Stc<T> &o, int k) {
+  const GA<T> dz{g.dz};
   int kp2 = imin(k + 2, g.nz), kp3 = imin(k + 3, g.nz);
-  double indzm = 1.0 / (dz[k] + dz[k - 1]), indzp = 1.0 / (dz[k] + dz[k + 1]);
-  double indzpp = 1.0 / (dz[k + 1] + dz[kp2]), indzppp = 1.0 / (dz[kp2] + dz[kp3]);
+  T indzm = T(1) / (dz[k] + dz[k - 1]), indzp = T(1) / (dz[k] + dz[k + 1]);
+  T indzpp = T(1) / (dz[k + 1] + dz[kp2]), indzppp = T(1) / (dz[kp2] + dz[kp3]);
   o.c = (dz[k + 1] * f.c + dz[k] * f.t) * indzp;
   o.e = (dz[k + 1] * f.e + dz[k] * f.et) * indzp;
   o.t = (dz[kp2] * f.t + dz[k + 1] * f.tt) * indzpp;
@@ -176,191 +185,193 @@ __device__ __forceinline__ void interp_z(const Grid &g, const Stc &f, Stc &o, in
 }
 
 // face fluxes of one cell: E, N, T always; W, S, B when set
+template <typename T>
 struct Faces {
-  double fe, fn, ft, fw, fs, fb;
+  T fe, fn, ft, fw, fs, fb;
   bool has_w, has_s, has_b;
 };
 
 // ----------------------------------------------------------------------
 // momentum along x, flux_u.f:41-371
-template <int D>
+template <int D, typename T>
 __device__ __forceinline__ void flux_u(const Grid &g, const Consts &k_, bool cw, bool cs_, bool cb,
-                                       const Stc &d, const Stc &u, const Stc &v, const Stc &w,
-                                       int i, int j, int k, bool muscl, Faces &o) {
-  const double *dx = g.dx;
-  double indxp = 1.0 / (dx[i] + dx[i + 1]), cs;
+                                       const Stc<T> &d, const Stc<T> &u, const Stc<T> &v, const Stc<T> &w,
+                                       int i, int j, int k, bool muscl, Faces<T> &o) {
+  const GA<T> dx{g.dx};
+  T indxp = T(1) / (dx[i] + dx[i + 1]), cs;
   o.has_w = !cw; o.has_s = (D == 3) && !cs_; o.has_b = !cb;
   if (D == 2) {
-    const double *r = g.r;
-    if (!cw) { cs = 0.5 * (u.c + u.w); o.fw = upw(cs) * d.c * u.c * r[i] + dnw(cs) * d.w * u.w * r[i]; }
+    const GA<T> r{g.r};
+    if (!cw) { cs = T(0.5) * (u.c + u.w); o.fw = upw(cs) * d.c * u.c * r[i] + dnw(cs) * d.w * u.w * r[i]; }
     if (!cb) { cs = (dx[i + 1] * w.b + dx[i] * w.eb) * indxp; o.fb = upw(cs) * d.c * u.c + dnw(cs) * d.b * u.b; }
-    cs = 0.5 * (u.c + u.e);
+    cs = T(0.5) * (u.c + u.e);
     o.fe = upw(cs) * d.e * u.e * r[i + 1] + dnw(cs) * d.c * u.c * r[i + 1];
     cs = (dx[i + 1] * w.c + dx[i] * w.e) * indxp;
     o.ft = upw(cs) * d.t * u.t + dnw(cs) * d.c * u.c;
     if (muscl) {
-      const double *dz = g.dz, *indx = g.indx;
+      const GA<T> dz{g.dz}, indx{g.indx};
       int ip2 = imin(i + 2, g.nx), kp2 = imin(k + 2, g.nz);
-      double indzm = 1.0 / (dz[k] + dz[k - 1]), indzp = 1.0 / (dz[k] + dz[k + 1]);
-      double indzpp = 1.0 / (dz[k + 1] + dz[kp2]);
-      double gradc = indx[i + 1] * (d.e * u.e - d.c * u.c);
-      double gradw = indx[i] * (d.c * u.c - d.w * u.w);
-      double grade = indx[ip2] * (d.ee * u.ee - d.e * u.e);
-      cs = 0.5 * (u.c + u.e);
-      o.fe = o.fe + muscl_incr(k_.lv, k_.beta, true, gradc, gradw, grade, cs, 0.5 * dx[i + 1], -0.5 * dx[i + 1]) * cs * r[i + 1];
-      gradc = (d.t * u.t - d.c * u.c) * 2.0 * indzp;
-      double gradb = (d.c * u.c - d.b * u.b) * 2.0 * indzm;
-      double gradt = (d.tt * u.tt - d.t * u.t) * 2.0 * indzpp;
+      T indzm = T(1) / (dz[k] + dz[k - 1]), indzp = T(1) / (dz[k] + dz[k + 1]);
+      T indzpp = T(1) / (dz[k + 1] + dz[kp2]);
+      T gradc = indx[i + 1] * (d.e * u.e - d.c * u.c);
+      T gradw = indx[i] * (d.c * u.c - d.w * u.w);
+      T grade = indx[ip2] * (d.ee * u.ee - d.e * u.e);
+      cs = T(0.5) * (u.c + u.e);
+      o.fe = o.fe + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradw, grade, cs, T(0.5) * dx[i + 1], -T(0.5) * dx[i + 1]) * cs * r[i + 1];
+      gradc = (d.t * u.t - d.c * u.c) * T(2) * indzp;
+      T gradb = (d.c * u.c - d.b * u.b) * T(2) * indzm;
+      T gradt = (d.tt * u.tt - d.t * u.t) * T(2) * indzpp;
       cs = (dx[i + 1] * w.c + dx[i] * w.e) * indxp;
-      o.ft = o.ft + muscl_incr(k_.lv, k_.beta, true, gradc, gradb, gradt, cs, 0.5 * dz[k], -0.5 * dz[k + 1]) * cs;
+      o.ft = o.ft + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradb, gradt, cs, T(0.5) * dz[k], -T(0.5) * dz[k + 1]) * cs;
     }
   } else {
-    if (!cw) { cs = 0.5 * (u.c + u.w); o.fw = upw(cs) * d.c * u.c + dnw(cs) * d.w * u.w; }
+    if (!cw) { cs = T(0.5) * (u.c + u.w); o.fw = upw(cs) * d.c * u.c + dnw(cs) * d.w * u.w; }
     if (!cs_) { cs = (dx[i + 1] * v.s + dx[i] * v.es) * indxp; o.fs = upw(cs) * d.c * u.c + dnw(cs) * d.s * u.s; }
     if (!cb) { cs = (dx[i + 1] * w.b + dx[i] * w.eb) * indxp; o.fb = upw(cs) * d.c * u.c + dnw(cs) * d.b * u.b; }
-    cs = 0.5 * (u.c + u.e);
+    cs = T(0.5) * (u.c + u.e);
     o.fe = upw(cs) * d.e * u.e + dnw(cs) * d.c * u.c;
     cs = (dx[i + 1] * v.c + dx[i] * v.e) * indxp;
     o.fn = upw(cs) * d.n * u.n + dnw(cs) * d.c * u.c;
     cs = (dx[i + 1] * w.c + dx[i] * w.e) * indxp;
     o.ft = upw(cs) * d.t * u.t + dnw(cs) * d.c * u.c;
     if (muscl) {
-      const double *dy = g.dy, *dz = g.dz, *indx = g.indx;
+      const GA<T> dy{g.dy}, dz{g.dz}, indx{g.indx};
       int ip2 = imin(i + 2, g.nx), jp2 = imin(j + 2, g.ny), kp2 = imin(k + 2, g.nz);
-      double indym = 1.0 / (dy[j] + dy[j - 1]), indyp = 1.0 / (dy[j] + dy[j + 1]);
-      double indypp = 1.0 / (dy[j + 1] + dy[jp2]);
-      double indzm = 1.0 / (dz[k] + dz[k - 1]), indzp = 1.0 / (dz[k] + dz[k + 1]);
-      double indzpp = 1.0 / (dz[k + 1] + dz[kp2]);
-      double gradw = indx[i] * (u.c * d.c - u.w * d.w);
-      double gradc = indx[i + 1] * (u.e * d.e - u.c * d.c);
-      double grade = indx[ip2] * (u.ee * d.ee - u.e * d.e);
-      cs = 0.5 * (u.c + u.e);
-      o.fe = o.fe + muscl_incr(k_.lv, k_.beta, true, gradc, gradw, grade, cs, 0.5 * dx[i + 1], -0.5 * dx[i + 1]) * cs;
-      double grads = (u.c * d.c - u.s * d.s) * 2.0 * indym;
-      gradc = (u.n * d.n - u.c * d.c) * 2.0 * indyp;
-      double gradn = (u.nn * d.nn - u.n * d.n) * 2.0 * indypp;
+      T indym = T(1) / (dy[j] + dy[j - 1]), indyp = T(1) / (dy[j] + dy[j + 1]);
+      T indypp = T(1) / (dy[j + 1] + dy[jp2]);
+      T indzm = T(1) / (dz[k] + dz[k - 1]), indzp = T(1) / (dz[k] + dz[k + 1]);
+      T indzpp = T(1) / (dz[k + 1] + dz[kp2]);
+      T gradw = indx[i] * (u.c * d.c - u.w * d.w);
+      T gradc = indx[i + 1] * (u.e * d.e - u.c * d.c);
+      T grade = indx[ip2] * (u.ee * d.ee - u.e * d.e);
+      cs = T(0.5) * (u.c + u.e);
+      o.fe = o.fe + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradw, grade, cs, T(0.5) * dx[i + 1], -T(0.5) * dx[i + 1]) * cs;
+      T grads = (u.c * d.c - u.s * d.s) * T(2) * indym;
+      gradc = (u.n * d.n - u.c * d.c) * T(2) * indyp;
+      T gradn = (u.nn * d.nn - u.n * d.n) * T(2) * indypp;
       cs = (dx[i + 1] * v.c + dx[i] * v.e) * indxp;
-      o.fn = o.fn + muscl_incr(k_.lv, k_.beta, true, gradc, grads, gradn, cs, 0.5 * dy[j], -0.5 * dy[j + 1]) * cs;
-      double gradb = (u.c * d.c - u.b * d.b) * 2.0 * indzm;
-      gradc = (u.t * d.t - u.c * d.c) * 2.0 * indzp;
-      double gradt = (u.tt * d.tt - u.t * d.t) * 2.0 * indzpp;
+      o.fn = o.fn + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, grads, gradn, cs, T(0.5) * dy[j], -T(0.5) * dy[j + 1]) * cs;
+      T gradb = (u.c * d.c - u.b * d.b) * T(2) * indzm;
+      gradc = (u.t * d.t - u.c * d.c) * T(2) * indzp;
+      T gradt = (u.tt * d.tt - u.t * d.t) * T(2) * indzpp;
       cs = (dx[i + 1] * w.c + dx[i] * w.e) * indxp;
-      o.ft = o.ft + muscl_incr(k_.lv, k_.beta, true, gradc, gradb, gradt, cs, 0.5 * dz[k], -0.5 * dz[k + 1]) * cs;
+      o.ft = o.ft + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradb, gradt, cs, T(0.5) * dz[k], -T(0.5) * dz[k + 1]) * cs;
     }
   }
 }
 
 // momentum along y (3D only), flux_v.f:41-222
+template <typename T>
 __device__ __forceinline__ void flux_v(const Grid &g, const Consts &k_, bool cw, bool cs_, bool cb,
-                                       const Stc &d, const Stc &u, const Stc &v, const Stc &w,
-                                       int i, int j, int k, bool muscl, Faces &o) {
-  const double *dy = g.dy;
-  double indyp = 1.0 / (dy[j] + dy[j + 1]), cs;
+                                       const Stc<T> &d, const Stc<T> &u, const Stc<T> &v, const Stc<T> &w,
+                                       int i, int j, int k, bool muscl, Faces<T> &o) {
+  const GA<T> dy{g.dy};
+  T indyp = T(1) / (dy[j] + dy[j + 1]), cs;
   o.has_w = !cw; o.has_s = !cs_; o.has_b = !cb;
   if (!cw) { cs = (u.wn * dy[j] + u.w * dy[j + 1]) * indyp; o.fw = upw(cs) * d.c * v.c + dnw(cs) * d.w * v.w; }
-  if (!cs_) { cs = 0.5 * (v.c + v.s); o.fs = upw(cs) * d.c * v.c + dnw(cs) * d.s * v.s; }
+  if (!cs_) { cs = T(0.5) * (v.c + v.s); o.fs = upw(cs) * d.c * v.c + dnw(cs) * d.s * v.s; }
   if (!cb) { cs = (w.nb * dy[j] + w.b * dy[j + 1]) * indyp; o.fb = upw(cs) * d.c * v.c + dnw(cs) * d.b * v.b; }
   cs = indyp * (u.c * dy[j + 1] + u.n * dy[j]);
   o.fe = upw(cs) * d.e * v.e + dnw(cs) * d.c * v.c;
-  cs = 0.5 * (v.n + v.c);
+  cs = T(0.5) * (v.n + v.c);
   o.fn = upw(cs) * d.n * v.n + dnw(cs) * d.c * v.c;
   cs = indyp * (w.c * dy[j + 1] + w.n * dy[j]);
   o.ft = upw(cs) * d.t * v.t + dnw(cs) * d.c * v.c;
   if (muscl) {
-    const double *dx = g.dx, *dz = g.dz, *indy = g.indy;
+    const GA<T> dx{g.dx}, dz{g.dz}, indy{g.indy};
     int ip2 = imin(i + 2, g.nx), jp2 = imin(j + 2, g.ny), kp2 = imin(k + 2, g.nz);
-    double indxm = 1.0 / (dx[i] + dx[i - 1]), indxp = 1.0 / (dx[i] + dx[i + 1]);
-    double indxpp = 1.0 / (dx[i + 1] + dx[ip2]);
-    double indzm = 1.0 / (dz[k] + dz[k - 1]), indzp = 1.0 / (dz[k] + dz[k + 1]);
-    double indzpp = 1.0 / (dz[k + 1] + dz[kp2]);
-    double gradw = 2.0 * indxm * (v.c * d.c - v.w * d.w);
-    double gradc = 2.0 * indxp * (v.e * d.e - v.c * d.c);
-    double grade = 2.0 * indxpp * (v.ee * d.ee - v.e * d.e);
+    T indxm = T(1) / (dx[i] + dx[i - 1]), indxp = T(1) / (dx[i] + dx[i + 1]);
+    T indxpp = T(1) / (dx[i + 1] + dx[ip2]);
+    T indzm = T(1) / (dz[k] + dz[k - 1]), indzp = T(1) / (dz[k] + dz[k + 1]);
+    T indzpp = T(1) / (dz[k + 1] + dz[kp2]);
+    T gradw = T(2) * indxm * (v.c * d.c - v.w * d.w);
+    T gradc = T(2) * indxp * (v.e * d.e - v.c * d.c);
+    T grade = T(2) * indxpp * (v.ee * d.ee - v.e * d.e);
     cs = indyp * (u.c * dy[j + 1] + u.n * dy[j]);
-    o.fe = o.fe + muscl_incr(k_.lv, k_.beta, true, gradc, gradw, grade, cs, 0.5 * dx[i], -0.5 * dx[i + 1]) * cs;
-    double grads = (v.c * d.c - v.s * d.s) * indy[j];
+    o.fe = o.fe + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradw, grade, cs, T(0.5) * dx[i], -T(0.5) * dx[i + 1]) * cs;
+    T grads = (v.c * d.c - v.s * d.s) * indy[j];
     gradc = (v.n * d.n - v.c * d.c) * indy[j + 1];
-    double gradn = (v.nn * d.nn - v.n * d.n) * indy[jp2];
-    cs = 0.5 * (v.n + v.c);
-    o.fn = o.fn + muscl_incr(k_.lv, k_.beta, true, gradc, grads, gradn, cs, 0.5 * dy[j + 1], -0.5 * dy[j + 1]) * cs;
-    double gradb = (v.c * d.c - v.b * d.b) * 2.0 * indzm;
-    gradc = (v.t * d.t - v.c * d.c) * 2.0 * indzp;
-    double gradt = (v.tt * d.tt - v.t * d.t) * 2.0 * indzpp;
+    T gradn = (v.nn * d.nn - v.n * d.n) * indy[jp2];
+    cs = T(0.5) * (v.n + v.c);
+    o.fn = o.fn + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, grads, gradn, cs, T(0.5) * dy[j + 1], -T(0.5) * dy[j + 1]) * cs;
+    T gradb = (v.c * d.c - v.b * d.b) * T(2) * indzm;
+    gradc = (v.t * d.t - v.c * d.c) * T(2) * indzp;
+    T gradt = (v.tt * d.tt - v.t * d.t) * T(2) * indzpp;
     cs = (dy[j + 1] * w.c + dy[j] * w.n) * indyp;
-    o.ft = o.ft + muscl_incr(k_.lv, k_.beta, true, gradc, gradb, gradt, cs, 0.5 * dz[k], -0.5 * dz[k + 1]) * cs;
+    o.ft = o.ft + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradb, gradt, cs, T(0.5) * dz[k], -T(0.5) * dz[k + 1]) * cs;
   }
 }
 
 // momentum along z, flux_w.f:40-356
-template <int D>
+template <int D, typename T>
 __device__ __forceinline__ void flux_w(const Grid &g, const Consts &k_, bool cw, bool cs_, bool cb,
-                                       const Stc &d, const Stc &u, const Stc &v, const Stc &w,
-                                       int i, int j, int k, bool muscl, Faces &o) {
-  const double *dz = g.dz;
-  double indzp = 1.0 / (dz[k] + dz[k + 1]), cs;
+                                       const Stc<T> &d, const Stc<T> &u, const Stc<T> &v, const Stc<T> &w,
+                                       int i, int j, int k, bool muscl, Faces<T> &o) {
+  const GA<T> dz{g.dz};
+  T indzp = T(1) / (dz[k] + dz[k + 1]), cs;
   o.has_w = !cw; o.has_s = (D == 3) && !cs_; o.has_b = !cb;
   if (D == 2) {
-    const double *rb = g.rb;
+    const GA<T> rb{g.rb};
     if (!cw) {
       cs = (u.wt * dz[k] + u.w * dz[k + 1]) * indzp;
-      o.fw = 0.5 * d.w * w.w * (cs + fabs(cs)) * rb[i - 1] + 0.5 * d.c * w.c * (cs - fabs(cs)) * rb[i - 1];
+      o.fw = T(0.5) * d.w * w.w * (cs + fabs(cs)) * rb[i - 1] + T(0.5) * d.c * w.c * (cs - fabs(cs)) * rb[i - 1];
     }
     if (!cb) {
-      cs = 0.5 * (w.c + w.b);
-      o.fb = 0.5 * d.b * w.b * (cs + fabs(cs)) + 0.5 * d.c * w.c * (cs - fabs(cs));
+      cs = T(0.5) * (w.c + w.b);
+      o.fb = T(0.5) * d.b * w.b * (cs + fabs(cs)) + T(0.5) * d.c * w.c * (cs - fabs(cs));
     }
     cs = indzp * (u.c * dz[k + 1] + u.t * dz[k]);
-    o.fe = 0.5 * d.c * w.c * (cs + fabs(cs)) * rb[i] + 0.5 * d.e * w.e * (cs - fabs(cs)) * rb[i];
-    cs = 0.5 * (w.t + w.c);
-    o.ft = 0.5 * d.c * w.c * (cs + fabs(cs)) + 0.5 * d.t * w.t * (cs - fabs(cs));
+    o.fe = T(0.5) * d.c * w.c * (cs + fabs(cs)) * rb[i] + T(0.5) * d.e * w.e * (cs - fabs(cs)) * rb[i];
+    cs = T(0.5) * (w.t + w.c);
+    o.ft = T(0.5) * d.c * w.c * (cs + fabs(cs)) + T(0.5) * d.t * w.t * (cs - fabs(cs));
     if (muscl) {
-      const double *dx = g.dx, *indz = g.indz;
+      const GA<T> dx{g.dx}, indz{g.indz};
       int ip2 = imin(i + 2, g.nx), kp2 = imin(k + 2, g.nz);
-      double indxm = 1.0 / (dx[i] + dx[i - 1]), indxp = 1.0 / (dx[i] + dx[i + 1]);
-      double indxpp = 1.0 / (dx[i + 1] + dx[ip2]);
-      double gradc = 2.0 * indxp * (d.e * w.e - d.c * w.c);
-      double gradw = 2.0 * indxm * (d.c * w.c - d.w * w.w);
-      double grade = 2.0 * indxpp * (d.ee * w.ee - d.e * w.e);
+      T indxm = T(1) / (dx[i] + dx[i - 1]), indxp = T(1) / (dx[i] + dx[i + 1]);
+      T indxpp = T(1) / (dx[i + 1] + dx[ip2]);
+      T gradc = T(2) * indxp * (d.e * w.e - d.c * w.c);
+      T gradw = T(2) * indxm * (d.c * w.c - d.w * w.w);
+      T grade = T(2) * indxpp * (d.ee * w.ee - d.e * w.e);
       cs = indzp * (u.c * dz[k + 1] + u.t * dz[k]);
-      o.fe = o.fe + muscl_incr(k_.lv, k_.beta, true, gradc, gradw, grade, cs, 0.5 * dx[i], -0.5 * dx[i + 1]) * cs * rb[i];
+      o.fe = o.fe + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradw, grade, cs, T(0.5) * dx[i], -T(0.5) * dx[i + 1]) * cs * rb[i];
       gradc = (d.t * w.t - d.c * w.c) * indz[k + 1];
-      double gradb = (d.c * w.c - d.b * w.b) * indz[k];
-      double gradt = (d.tt * w.tt - d.t * w.t) * indz[kp2];
-      cs = 0.5 * (w.t + w.c);
-      o.ft = o.ft + muscl_incr(k_.lv, k_.beta, true, gradc, gradb, gradt, cs, 0.5 * dz[k + 1], -0.5 * dz[k + 1]) * cs;
+      T gradb = (d.c * w.c - d.b * w.b) * indz[k];
+      T gradt = (d.tt * w.tt - d.t * w.t) * indz[kp2];
+      cs = T(0.5) * (w.t + w.c);
+      o.ft = o.ft + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradb, gradt, cs, T(0.5) * dz[k + 1], -T(0.5) * dz[k + 1]) * cs;
     }
   } else {
-    if (!cw) { cs = (dz[k + 1] * u.w + dz[k] * u.wt) * indzp; o.fw = 0.5 * d.w * w.w * (cs + fabs(cs)) + 0.5 * d.c * w.c * (cs - fabs(cs)); }
-    if (!cs_) { cs = (dz[k + 1] * v.s + dz[k] * v.st) * indzp; o.fs = 0.5 * d.s * w.s * (cs + fabs(cs)) + 0.5 * d.c * w.c * (cs - fabs(cs)); }
-    if (!cb) { cs = 0.5 * (w.b + w.c); o.fb = 0.5 * d.b * w.b * (cs + fabs(cs)) + 0.5 * d.c * w.c * (cs - fabs(cs)); }
+    if (!cw) { cs = (dz[k + 1] * u.w + dz[k] * u.wt) * indzp; o.fw = T(0.5) * d.w * w.w * (cs + fabs(cs)) + T(0.5) * d.c * w.c * (cs - fabs(cs)); }
+    if (!cs_) { cs = (dz[k + 1] * v.s + dz[k] * v.st) * indzp; o.fs = T(0.5) * d.s * w.s * (cs + fabs(cs)) + T(0.5) * d.c * w.c * (cs - fabs(cs)); }
+    if (!cb) { cs = T(0.5) * (w.b + w.c); o.fb = T(0.5) * d.b * w.b * (cs + fabs(cs)) + T(0.5) * d.c * w.c * (cs - fabs(cs)); }
     cs = (dz[k + 1] * u.c + dz[k] * u.t) * indzp;
-    o.fe = 0.5 * d.c * w.c * (cs + fabs(cs)) + 0.5 * d.e * w.e * (cs - fabs(cs));
+    o.fe = T(0.5) * d.c * w.c * (cs + fabs(cs)) + T(0.5) * d.e * w.e * (cs - fabs(cs));
     cs = (dz[k + 1] * v.c + dz[k] * v.t) * indzp;
-    o.fn = 0.5 * d.c * w.c * (cs + fabs(cs)) + 0.5 * d.n * w.n * (cs - fabs(cs));
-    cs = 0.5 * (w.c + w.t);
-    o.ft = 0.5 * d.c * w.c * (cs + fabs(cs)) + 0.5 * d.t * w.t * (cs - fabs(cs));
+    o.fn = T(0.5) * d.c * w.c * (cs + fabs(cs)) + T(0.5) * d.n * w.n * (cs - fabs(cs));
+    cs = T(0.5) * (w.c + w.t);
+    o.ft = T(0.5) * d.c * w.c * (cs + fabs(cs)) + T(0.5) * d.t * w.t * (cs - fabs(cs));
     if (muscl) {
-      const double *dx = g.dx, *dy = g.dy, *indz = g.indz;
+      const GA<T> dx{g.dx}, dy{g.dy}, indz{g.indz};
       int ip2 = imin(i + 2, g.nx), jp2 = imin(j + 2, g.ny), kp2 = imin(k + 2, g.nz);
-      double indxm = 1.0 / (dx[i] + dx[i - 1]), indxp = 1.0 / (dx[i] + dx[i + 1]);
-      double indxpp = 1.0 / (dx[i + 1] + dx[ip2]);
-      double indym = 1.0 / (dy[j] + dy[j - 1]), indyp = 1.0 / (dy[j] + dy[j + 1]);
-      double indypp = 1.0 / (dy[j + 1] + dy[jp2]);
-      double gradw = 2.0 * indxm * (w.c * d.c - w.w * d.w);
-      double gradc = 2.0 * indxp * (w.e * d.e - w.c * d.c);
-      double grade = 2.0 * indxpp * (w.ee * d.ee - w.e * d.e);
+      T indxm = T(1) / (dx[i] + dx[i - 1]), indxp = T(1) / (dx[i] + dx[i + 1]);
+      T indxpp = T(1) / (dx[i + 1] + dx[ip2]);
+      T indym = T(1) / (dy[j] + dy[j - 1]), indyp = T(1) / (dy[j] + dy[j + 1]);
+      T indypp = T(1) / (dy[j + 1] + dy[jp2]);
+      T gradw = T(2) * indxm * (w.c * d.c - w.w * d.w);
+      T gradc = T(2) * indxp * (w.e * d.e - w.c * d.c);
+      T grade = T(2) * indxpp * (w.ee * d.ee - w.e * d.e);
       cs = (dz[k + 1] * u.c + dz[k] * u.t) * indzp;
-      o.fe = o.fe + muscl_incr(k_.lv, k_.beta, true, gradc, gradw, grade, cs, 0.5 * dx[i], -0.5 * dx[i + 1]) * cs;
-      double grads = 2.0 * indym * (w.c * d.c - w.s * d.s);
-      gradc = 2.0 * indyp * (w.n * d.n - w.c * d.c);
-      double gradn = 2.0 * indypp * (w.nn * d.nn - w.n * d.n);
+      o.fe = o.fe + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradw, grade, cs, T(0.5) * dx[i], -T(0.5) * dx[i + 1]) * cs;
+      T grads = T(2) * indym * (w.c * d.c - w.s * d.s);
+      gradc = T(2) * indyp * (w.n * d.n - w.c * d.c);
+      T gradn = T(2) * indypp * (w.nn * d.nn - w.n * d.n);
       cs = (dz[k + 1] * v.c + dz[k] * v.t) * indzp;
-      o.fn = o.fn + muscl_incr(k_.lv, k_.beta, true, gradc, grads, gradn, cs, 0.5 * dy[j], -0.5 * dy[j + 1]) * cs;
-      double gradb = indz[k] * (w.c * d.c - w.b * d.b);
+      o.fn = o.fn + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, grads, gradn, cs, T(0.5) * dy[j], -T(0.5) * dy[j + 1]) * cs;
+      T gradb = indz[k] * (w.c * d.c - w.b * d.b);
       gradc = indz[k + 1] * (w.t * d.t - w.c * d.c);
-      double gradt = indz[kp2] * (w.tt * d.tt - w.t * d.t);
-      cs = 0.5 * (w.c + w.t);
-      o.ft = o.ft + muscl_incr(k_.lv, k_.beta, true, gradc, gradb, gradt, cs, 0.5 * dz[k + 1], -0.5 * dz[k + 1]) * cs;
+      T gradt = indz[kp2] * (w.tt * d.tt - w.t * d.t);
+      cs = T(0.5) * (w.c + w.t);
+      o.ft = o.ft + muscl_incr<T>(k_.lv, T(k_.beta), true, gradc, gradb, gradt, cs, T(0.5) * dz[k + 1], -T(0.5) * dz[k + 1]) * cs;
     }
   }
 }

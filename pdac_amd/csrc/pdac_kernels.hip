@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(kBlock) k_copy(double *dst, const double *src,
 // fluxes of one phase (blockIdx.y) of each computed cell.  E/N/T faces are
 // owned by the cell; a W/S/B face is written only when that neighbour is
 // not computed (then no other cell writes it).
-template <int D, int NS>
+template <int D, int NS, typename FT>
 __global__ void __launch_bounds__(kBlock) k_momflux(Grid g, Consts K, Fields F) {
   long c = cell_id();
   if (c >= g.n) return;
@@ -472,7 +472,7 @@ __global__ void __launch_bounds__(kBlock) k_momflux(Grid g, Consts K, Fields F) 
     long o = (long)(ph - 1) * n;
     uu = F.f[PDAC_US] + o; vv = F.f[PDAC_VS] + o; ww = F.f[PDAC_WS] + o; dd = F.f[PDAC_RLK] + o;
   }
-  Stc u, v, w, dens, sx, sy, sz;
+  Stc<FT> u, v, w, dens, sx, sy, sz;
   gather_rnb<D>(x, uu, u);
   gather_rnb<D>(x, ww, w);
   if (D == 3) gather_rnb<D>(x, vv, v);
@@ -485,20 +485,25 @@ __global__ void __launch_bounds__(kBlock) k_momflux(Grid g, Consts K, Fields F) 
   bool cs_ = (D == 3) ? computed(fl[x.s]) : true;
   bool mus = K.muscl > 0 && fc == FLUID;
   long o = (long)ph * n;
-  Faces fu, fv, fw;
+  Faces<FT> fu, fv, fw;
   flux_u<D>(g, K, cw, cs_, cb, sx, u, v, w, i, j, k, mus && i != g.nx - 1, fu);
   flux_w<D>(g, K, cw, cs_, cb, sz, u, v, w, i, j, k, mus && k != g.nz - 1, fw);
-  F.mf[0][0][o + c] = fu.fe; F.mf[0][2][o + c] = fu.ft;
-  F.mf[2][0][o + c] = fw.fe; F.mf[2][2][o + c] = fw.ft;
-  if (fu.has_w) { F.mf[0][0][o + x.w] = fu.fw; F.mf[2][0][o + x.w] = fw.fw; }
-  if (fu.has_b) { F.mf[0][2][o + x.b] = fu.fb; F.mf[2][2][o + x.b] = fw.fb; }
+  FT *m[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) m[a][b] = (FT *)F.mf[a][b];
+  m[0][0][o + c] = fu.fe; m[0][2][o + c] = fu.ft;
+  m[2][0][o + c] = fw.fe; m[2][2][o + c] = fw.ft;
+  if (fu.has_w) { m[0][0][o + x.w] = fu.fw; m[2][0][o + x.w] = fw.fw; }
+  if (fu.has_b) { m[0][2][o + x.b] = fu.fb; m[2][2][o + x.b] = fw.fb; }
   if (D == 3) {
     flux_v(g, K, cw, cs_, cb, sy, u, v, w, i, j, k, mus && j != g.ny - 1, fv);
-    F.mf[0][1][o + c] = fu.fn; F.mf[1][0][o + c] = fv.fe; F.mf[1][1][o + c] = fv.fn;
-    F.mf[1][2][o + c] = fv.ft; F.mf[2][1][o + c] = fw.fn;
-    if (fu.has_w) F.mf[1][0][o + x.w] = fv.fw;
-    if (fu.has_b) F.mf[1][2][o + x.b] = fv.fb;
-    if (fu.has_s) { F.mf[0][1][o + x.s] = fu.fs; F.mf[1][1][o + x.s] = fv.fs; F.mf[2][1][o + x.s] = fw.fs; }
+    m[0][1][o + c] = fu.fn; m[1][0][o + c] = fv.fe; m[1][1][o + c] = fv.fn;
+    m[1][2][o + c] = fv.ft; m[2][1][o + c] = fw.fn;
+    if (fu.has_w) m[1][0][o + x.w] = fv.fw;
+    if (fu.has_b) m[1][2][o + x.b] = fv.fb;
+    if (fu.has_s) { m[0][1][o + x.s] = fu.fs; m[1][1][o + x.s] = fv.fs; m[2][1][o + x.s] = fw.fs; }
   }
 }
 
@@ -554,7 +559,7 @@ __device__ __forceinline__ void inter_d(const Consts &K, const double *kpgv, dou
 // interphase matrix and the (1-alpha) coupling.  Neighbour interphase
 // entries come from the copy taken before this call (the serial loop reads
 // east/north/top neighbours before it reaches them).
-template <int D, int NS>
+template <int D, int NS, typename FT>
 __global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, double dt, int old_zero) {
   long c = cell_id();
   if (c >= g.n) return;
@@ -566,22 +571,27 @@ __global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, do
   decode(g, c, i, j, k);
   Nbr<D> q;
   first_nbrs<D>(g, i, j, k, q);
-  double *const(*mf)[3] = F.mf;
+  // momentum fluxes (stored in FT); their differences are taken in double
+  const FT *mf[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) mf[a][b] = (const FT *)F.mf[a][b];
   const double *dx = g.dx, *dy = g.dy, *dz = g.dz;
   const double *rgpn = F.f[PDAC_RGPN], *rlkn = F.f[PDAC_RLKN], *pn = F.f[PDAC_PN], *epn = F.f[PDAC_EPN];
   const double *ug = F.f[PDAC_UG], *vg = F.f[PDAC_VG], *wg = F.f[PDAC_WG];
   const double *us = F.f[PDAC_US], *vs = F.f[PDAC_VS], *ws = F.f[PDAC_WS];
-  double ugfx = mf[0][0][c] - mf[0][0][q.imjk];
-  double ugfz = mf[0][2][c] - mf[0][2][q.ijkm];
-  double wgfx = mf[2][0][c] - mf[2][0][q.imjk];
-  double wgfz = mf[2][2][c] - mf[2][2][q.ijkm];
+  double ugfx = (double)mf[0][0][c] - (double)mf[0][0][q.imjk];
+  double ugfz = (double)mf[0][2][c] - (double)mf[0][2][q.ijkm];
+  double wgfx = (double)mf[2][0][c] - (double)mf[2][0][q.imjk];
+  double wgfz = (double)mf[2][2][c] - (double)mf[2][2][q.ijkm];
   double ugfy = 0.0, vgfx = 0.0, vgfy = 0.0, vgfz = 0.0, wgfy = 0.0;
   if (D == 3) {
-    ugfy = mf[0][1][c] - mf[0][1][q.ijmk];
-    vgfx = mf[1][0][c] - mf[1][0][q.imjk];
-    vgfy = mf[1][1][c] - mf[1][1][q.ijmk];
-    vgfz = mf[1][2][c] - mf[1][2][q.ijkm];
-    wgfy = mf[2][1][c] - mf[2][1][q.ijmk];
+    ugfy = (double)mf[0][1][c] - (double)mf[0][1][q.ijmk];
+    vgfx = (double)mf[1][0][c] - (double)mf[1][0][q.imjk];
+    vgfy = (double)mf[1][1][c] - (double)mf[1][1][q.ijmk];
+    vgfz = (double)mf[1][2][c] - (double)mf[1][2][q.ijkm];
+    wgfy = (double)mf[2][1][c] - (double)mf[2][1][q.ijmk];
   }
   double dxi = dx[i], dxip1 = dx[i + 1], dyj = dy[j], dyjp1 = dy[j + 1];
   double dzk = dz[k], dzkp1 = dz[k + 1];
@@ -626,17 +636,17 @@ __global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, do
   for (int is = 0; is < NS; ++is) {
     long o = (long)(is + 1) * n, os = (long)is * n;
     rlkc[is] = rlk[os + c];
-    double usfx = mf[0][0][o + c] - mf[0][0][o + q.imjk];
-    double usfz = mf[0][2][o + c] - mf[0][2][o + q.ijkm];
-    double wsfx = mf[2][0][o + c] - mf[2][0][o + q.imjk];
-    double wsfz = mf[2][2][o + c] - mf[2][2][o + q.ijkm];
+    double usfx = (double)mf[0][0][o + c] - (double)mf[0][0][o + q.imjk];
+    double usfz = (double)mf[0][2][o + c] - (double)mf[0][2][o + q.ijkm];
+    double wsfx = (double)mf[2][0][o + c] - (double)mf[2][0][o + q.imjk];
+    double wsfz = (double)mf[2][2][o + c] - (double)mf[2][2][o + q.ijkm];
     double usfy = 0.0, vsfx = 0.0, vsfy = 0.0, vsfz = 0.0, wsfy = 0.0;
     if (D == 3) {
-      usfy = mf[0][1][o + c] - mf[0][1][o + q.ijmk];
-      vsfx = mf[1][0][o + c] - mf[1][0][o + q.imjk];
-      vsfy = mf[1][1][o + c] - mf[1][1][o + q.ijmk];
-      vsfz = mf[1][2][o + c] - mf[1][2][o + q.ijkm];
-      wsfy = mf[2][1][o + c] - mf[2][1][o + q.ijmk];
+      usfy = (double)mf[0][1][o + c] - (double)mf[0][1][o + q.ijmk];
+      vsfx = (double)mf[1][0][o + c] - (double)mf[1][0][o + q.imjk];
+      vsfy = (double)mf[1][1][o + c] - (double)mf[1][1][o + q.ijmk];
+      vsfz = (double)mf[1][2][o + c] - (double)mf[1][2][o + q.ijkm];
+      wsfy = (double)mf[2][1][o + c] - (double)mf[2][1][o + q.ijmk];
     }
     double epsn_e, epsn_n, epsn_t;
     if (K.pmodel == 1) {
