@@ -42,9 +42,15 @@ extern "C" {
 #define PDAC_PREC_FP64 0      /* plain FP64 drop-in */
 #define PDAC_PREC_FP32FLUX 1  /* MUSCL/limiter/flux arithmetic in FP32 */
 
-/* iteration orders of the pressure-velocity sweep */
-#define PDAC_ORDER_LEXICOGRAPHIC 0 /* reference Gauss-Seidel order (CPU only) */
-#define PDAC_ORDER_REDBLACK 1      /* two-colour order used on the GPU */
+/* Cell orders.  iter: the reference's Gauss-Seidel sweep (iter.f:241) or
+ * the two-colour sweep of the GPU.  Explicit step: the reference's serial
+ * stage loop (explicit_solver.f:1047-1062, whose gas-enthalpy pressure work
+ * reads already updated west/south/bottom neighbours) unless
+ * PDAC_ORDER_STAGE asks for every right-hand side of a stage from the
+ * stage's starting state. */
+#define PDAC_ORDER_LEXICOGRAPHIC 0 /* iter: reference order (CPU only); explicit: reference order */
+#define PDAC_ORDER_REDBLACK 1      /* iter: two-colour order (GPU); explicit: reference order */
+#define PDAC_ORDER_STAGE 2         /* explicit step only: stage-start order */
 
 typedef struct pdac_dims {
   int32_t job_type;  /* 2 or 3 (control_flags JOB_TYPE_2D/3D) */
@@ -78,7 +84,7 @@ typedef struct pdac_params {
   int32_t pmodel, drag_model, icpc, ord;
   int32_t cstop;                    /* stop when iter does not converge */
   int32_t iter_order;               /* PDAC_ORDER_* */
-  int32_t pad_;
+  int32_t time_integration;         /* 0 semi-implicit, 1 explicit (prog.f:192-268) */
 } pdac_params;
 
 /* Field identifiers for pdac_put/pdac_get.  ncomp is 1 unless noted
@@ -143,10 +149,22 @@ int pdac_ftem(pdac_ctx *ctx, double dt, int32_t *eos_fail); /* ftem.f:19 */
  * tilde.f:56-103); pdac_step calls it once per step. */
 int pdac_begin_step(pdac_ctx *ctx);
 
-/* nsteps full semi-implicit steps of main_prog (prog.f:143-329):
- * boundary, fieldn, then for rk = 1..rungekut with dt/(rungekut+1-rk):
- * tilde, iter, ygas, htilde, ftem.  nit_last receives the sweep count of the
- * last iter call, nit_total the sum over all calls. */
+/* The explicit step (time_integration = 1), explicit_solver.f.  The
+ * context needs pdac_params.time_integration = 1 at pdac_set_params, which
+ * then allocates the Runge-Kutta workspace.  explicit_iter returns the
+ * number of caloric-EOS failures (explicit_solver.f:1217-1219: more than one
+ * is PDAC_ST_EOS_FAILURE) and the largest Courant number over the computed
+ * cells (compute_courant :2063-2103). */
+int pdac_initialize_fieldn(pdac_ctx *ctx, double dt);   /* explicit_solver.f:419 */
+int pdac_explicit_iter(pdac_ctx *ctx, double dt, int32_t *eos_fail,
+                       double *cfl);                    /* explicit_solver.f:996 */
+
+/* nsteps full steps of main_prog (prog.f:143-329): boundary, then
+ * semi-implicit: fieldn and for rk = 1..rungekut with dt/(rungekut+1-rk):
+ * tilde, iter, ygas, htilde, ftem; explicit (time_integration = 1):
+ * initialize_fieldn and per stage explicit_iter, ygas.  nit_last receives the
+ * sweep count of the last iter call, nit_total the sum over all calls (0 for
+ * explicit steps). */
 int pdac_step(pdac_ctx *ctx, double dt, int32_t rungekut, int32_t nsteps,
               int32_t *nit_last, int64_t *nit_total);
 
@@ -166,7 +184,8 @@ enum pdac_kernel {
   PDAC_K_BOUNDARY = 0, PDAC_K_FIELDN, PDAC_K_MOMFLUX, PDAC_K_TILDE, PDAC_K_PREDICT,
   PDAC_K_GASFLUX, PDAC_K_SNAPSHOT, PDAC_K_SWEEP, PDAC_K_YGAS_FLUX, PDAC_K_YGAS_UPDATE,
   PDAC_K_HTILDE_FLUX, PDAC_K_HTILDE_UPDATE, PDAC_K_FTEM, PDAC_K_FILL, PDAC_K_SWEEP_CHECK,
-  PDAC_K_ITERATE, PDAC_K_COUNT
+  PDAC_K_ITERATE, PDAC_K_XINIT, PDAC_K_XFLUX, PDAC_K_XRHS, PDAC_K_XCHAIN, PDAC_K_XMASS,
+  PDAC_K_XSOLVE, PDAC_K_COUNT
 };
 int pdac_profile_enable(pdac_ctx *ctx, int enable);
 /* bit k of mask: time kernel k (default: all) */

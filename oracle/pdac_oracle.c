@@ -70,8 +70,11 @@ struct pdac_cpu_ctx {
   double *egf[3], *esf[3], *yf[3];
   double *conv, *abeta, *appu_old[3];
   double *snap_rgp, *snap_rlk;
+  struct xstate *xs; /* explicit Runge-Kutta state, allocated on first use */
   int grid_set, params_set;
 };
+
+static void x_free(struct xstate *x);
 
 static int ncomp_of(const pdac_cpu_ctx *s, int id) {
   switch (id) {
@@ -154,6 +157,7 @@ int pdac_cpu_destroy(pdac_cpu_ctx *s) {
   free(s->dx); free(s->indx); free(s->r); free(s->rb); free(s->inr);
   free(s->inrb); free(s->dy); free(s->indy); free(s->dz); free(s->indz);
   free(s->p_atm); free(s->t_atm); free(s->flag);
+  x_free(s->xs);
   free(s);
   return 0;
 }
@@ -693,3 +697,4 @@ void pdac_cpu_gauss_jordan(int np, const double *plim, double *a, double *b) {
 }
 
 #include "pdac_oracle_step.inc"
+#include "pdac_oracle_explicit.inc"

@@ -37,6 +37,12 @@ int pdac_cpu_iter(pdac_cpu_ctx *ctx, double dt, int32_t *nit, int32_t *converged
 int pdac_cpu_ygas(pdac_cpu_ctx *ctx, double dt);
 int pdac_cpu_htilde(pdac_cpu_ctx *ctx, double dt);
 int pdac_cpu_ftem(pdac_cpu_ctx *ctx, double dt, int32_t *eos_fail);
+int pdac_cpu_initialize_fieldn(pdac_cpu_ctx *ctx, double dt);
+int pdac_cpu_explicit_iter(pdac_cpu_ctx *ctx, double dt, int32_t *eos_fail, double *cfl);
+/* how calc_all_fluxes reads the west/south/bottom second neighbours of the
+ * mass-flux stencil, which nb leaves unset: 1 = as 0, 0 = the reflected
+ * second neighbours */
+int pdac_cpu_set_explicit_stale(pdac_cpu_ctx *ctx, int zero);
 int pdac_cpu_step(pdac_cpu_ctx *ctx, double dt, int32_t rungekut, int32_t nsteps,
                   int32_t *nit_last, int64_t *nit_total);
 

@@ -12,6 +12,7 @@ PREC_FP64 = 0
 PREC_FP32FLUX = 1
 ORDER_LEXICOGRAPHIC = 0
 ORDER_REDBLACK = 1
+ORDER_STAGE = 2
 
 ST_ITER_NOT_CONVERGED = 1
 ST_EOS_FAILURE = 2
@@ -58,7 +59,7 @@ class Params(ct.Structure):
         ("inmax", ct.c_int32), ("maxout", ct.c_int32),
         ("pmodel", ct.c_int32), ("drag_model", ct.c_int32), ("icpc", ct.c_int32),
         ("ord", ct.c_int32), ("cstop", ct.c_int32), ("iter_order", ct.c_int32),
-        ("pad_", ct.c_int32),
+        ("time_integration", ct.c_int32),
     ]
 
 
@@ -91,6 +92,7 @@ def params_from_case(case, iter_order=ORDER_REDBLACK) -> Params:
     p.pmodel, p.drag_model, p.icpc, p.ord = case.pmodel, case.drag_model, case.icpc, case.ord
     p.cstop = 1 if case.cstop else 0
     p.iter_order = iter_order
+    p.time_integration = case.time_integration
     return p
 
 

@@ -454,24 +454,15 @@ __global__ void __launch_bounds__(kBlock) k_copy(double *dst, const double *src,
 // fluxes of one phase (blockIdx.y) of each computed cell.  E/N/T faces are
 // owned by the cell; a W/S/B face is written only when that neighbour is
 // not computed (then no other cell writes it).
-template <int D, int NS, typename FT>
-__global__ void __launch_bounds__(kBlock) k_momflux(Grid g, Consts K, Fields F) {
-  long c = cell_id();
-  if (c >= g.n) return;
-  int fc = g.flag[c];
-  if (!computed(fc)) return;
-  int ph = blockIdx.y;
-  long n = g.n;
+template <int D, typename FT>
+__device__ __forceinline__ void momflux_cell(const Grid &g, const Consts &K, const Fields &F, long c,
+                                             int fc, int ph, const double *uu, const double *vv,
+                                             const double *ww, const double *dd) {
+  const long n = g.n;
   int i, j, k;
   decode(g, c, i, j, k);
   FullIdx<D> x;
   full_idx<D>(g, i, j, k, x);
-  const double *uu, *vv, *ww, *dd;
-  if (ph == 0) { uu = F.f[PDAC_UG]; vv = F.f[PDAC_VG]; ww = F.f[PDAC_WG]; dd = F.f[PDAC_RGP]; }
-  else {
-    long o = (long)(ph - 1) * n;
-    uu = F.f[PDAC_US] + o; vv = F.f[PDAC_VS] + o; ww = F.f[PDAC_WS] + o; dd = F.f[PDAC_RLK] + o;
-  }
   Stc<FT> u, v, w, dens, sx, sy, sz;
   gather_rnb<D>(x, uu, u);
   gather_rnb<D>(x, ww, w);
@@ -505,6 +496,22 @@ __global__ void __launch_bounds__(kBlock) k_momflux(Grid g, Consts K, Fields F) 
     if (fu.has_b) m[1][2][o + x.b] = fv.fb;
     if (fu.has_s) { m[0][1][o + x.s] = fu.fs; m[1][1][o + x.s] = fv.fs; m[2][1][o + x.s] = fw.fs; }
   }
+}
+
+template <int D, int NS, typename FT>
+__global__ void __launch_bounds__(kBlock) k_momflux(Grid g, Consts K, Fields F) {
+  long c = cell_id();
+  if (c >= g.n) return;
+  int fc = g.flag[c];
+  if (!computed(fc)) return;
+  int ph = blockIdx.y;
+  const double *uu, *vv, *ww, *dd;
+  if (ph == 0) { uu = F.f[PDAC_UG]; vv = F.f[PDAC_VG]; ww = F.f[PDAC_WG]; dd = F.f[PDAC_RGP]; }
+  else {
+    long o = (long)(ph - 1) * g.n;
+    uu = F.f[PDAC_US] + o; vv = F.f[PDAC_VS] + o; ww = F.f[PDAC_WS] + o; dd = F.f[PDAC_RLK] + o;
+  }
+  momflux_cell<D, FT>(g, K, F, c, fc, ph, uu, vv, ww, dd);
 }
 
 // inter, drag.f:173-246: packed symmetric interphase matrix with zero row sums
@@ -771,4 +778,5 @@ __global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, do
 
 #include "pdac_iter.hip.inc"
 #include "pdac_energy.hip.inc"
+#include "pdac_explicit.hip.inc"
 #include "pdac_api.hip.inc"

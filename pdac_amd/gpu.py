@@ -23,7 +23,7 @@ EXPORTS = [
     "pdac_tilde", "pdac_iter", "pdac_ygas", "pdac_htilde", "pdac_ftem",
     "pdac_begin_step", "pdac_step", "pdac_sync", "pdac_last_error", "pdac_get_momflux",
     "pdac_profile_enable", "pdac_profile_select", "pdac_profile_reset", "pdac_profile_read",
-    "pdac_profile_visits",
+    "pdac_profile_visits", "pdac_initialize_fieldn", "pdac_explicit_iter",
 ]
 
 
@@ -59,7 +59,10 @@ class GpuSolver(Solver):
         return out.reshape(self.case.nsolid + 1, self.n)
 
 
-def gpu_solver(case, device: int = 0, **kw) -> GpuSolver:
+def gpu_solver(case, device: int = 0, iter_order=abi.ORDER_REDBLACK, **kw) -> GpuSolver:
+    """GPU solver of ``case``.  iter_order: the red-black sweep of iter; for
+    an explicit case (time_integration = 1) the reference's stage order, or
+    ``abi.ORDER_STAGE``."""
     L = lib()
     create = lambda ctx, dims: L.pdac_create(ctx, dims, device)  # noqa: E731
-    return GpuSolver(L, "pdac_", create, case, iter_order=abi.ORDER_REDBLACK, **kw)
+    return GpuSolver(L, "pdac_", create, case, iter_order=iter_order, **kw)

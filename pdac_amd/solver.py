@@ -39,6 +39,8 @@ def bind(lib: ct.CDLL, prefix: str):
         "iter": [P, ct.c_double, ip, ip], "ygas": [P, ct.c_double],
         "htilde": [P, ct.c_double], "ftem": [P, ct.c_double, ip],
         "step": [P, ct.c_double, ct.c_int32, ct.c_int32, ip, ct.POINTER(ct.c_int64)],
+        "initialize_fieldn": [P, ct.c_double],
+        "explicit_iter": [P, ct.c_double, ip, ct.POINTER(ct.c_double)],
     }
     out = {}
     for name, args in sig.items():
@@ -171,8 +173,19 @@ class Solver:
         self._check("ftem", self.fn["ftem"](self.ctx, dt, ct.byref(fail)))
         return fail.value
 
+    def initialize_fieldn(self, dt):
+        self._check("initialize_fieldn", self.fn["initialize_fieldn"](self.ctx, dt))
+
+    def explicit_iter(self, dt):
+        """One explicit Runge-Kutta update (explicit_solver.f:996); returns
+        the equation-of-state failure count and the largest Courant number."""
+        fail, cfl = ct.c_int32(), ct.c_double()
+        self._check("explicit_iter", self.fn["explicit_iter"](self.ctx, dt, ct.byref(fail), ct.byref(cfl)))
+        return fail.value, cfl.value
+
     def step(self, nsteps=1, dt=None, rungekut=None):
-        """Advance ``nsteps`` semi-implicit steps in one library call."""
+        """Advance ``nsteps`` steps in one library call (explicit ones when the
+        case sets ``time_integration = 1``)."""
         dt = self.case.dt if dt is None else dt
         rk = self.case.rungekut if rungekut is None else rungekut
         nl, nt = ct.c_int32(), ct.c_int64()
@@ -187,6 +200,15 @@ class Solver:
         """One step as separate subroutine calls, in main_prog's order."""
         dt = self.case.dt if dt is None else dt
         rk_n = self.case.rungekut if rungekut is None else rungekut
+        if self.case.time_integration == 1:
+            self.boundary()
+            self.initialize_fieldn(dt)
+            for rk in range(1, rk_n + 1):
+                dts = dt / (rk_n + 1 - rk)
+                self.explicit_iter(dts)
+                self.ygas(dts)
+            self.time = self.time + dt
+            return 0
         self.boundary()
         self.fieldn()
         self.begin_step()

@@ -22,6 +22,13 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from pdac_amd.cases import eruption_column, mixed_box, shock_tube  # noqa: E402
 from tests.golden.refrun import available, run_reference  # noqa: E402
 
+def explicit(case):
+    """The same case advanced by the explicit Runge-Kutta solver
+    (time_integration = 1, explicit_solver.f)."""
+    case.time_integration = 1
+    return case
+
+
 CASES = {
     # 2D Cartesian shock tube (slip walls), one class, MUSCL + minmod
     "shock_tube_2d": lambda: shock_tube(nx=40, nz=8, nsteps=4, dt=1.0e-5, rungekut=1),
@@ -44,6 +51,19 @@ CASES = {
     # small eruption column: vent inflow, stratified atmosphere, 2 classes
     "eruption_2d_small": lambda: eruption_column(nx=24, nz=40, nsteps=3, dt=2.0e-3,
                                                  vent_radius=6.0),
+    # explicit Runge-Kutta steps (prog.f:192-268 with time_integration = 1)
+    "explicit_mixed_2d": lambda: explicit(mixed_box(nsteps=3, bottom=3)),
+    "explicit_mixed_2d_cyl_freeio": lambda: explicit(mixed_box(
+        nsteps=3, itc=1, west=2, east=4, bottom=3, top=2, stratification=True)),
+    "explicit_mixed_2d_variants": lambda: explicit(mixed_box(
+        nsteps=2, bottom=3, drag_model=2, pmodel=1, alpha=0.5, alphagrav=0.5, rungekut=2)),
+    "explicit_mixed_3d": lambda: explicit(mixed_box(
+        three_d=True, nsteps=2, nsolid=3, bottom=3, west=3, east=4, south=4, north=3, top=2,
+        stratification=True)),
+    "explicit_shock_tube_2d_rk3": lambda: explicit(shock_tube(nx=40, nz=8, nsteps=2, dt=1.0e-5,
+                                                              rungekut=3)),
+    "explicit_eruption_2d_small": lambda: explicit(eruption_column(nx=24, nz=40, nsteps=3,
+                                                                   dt=2.0e-4, vent_radius=6.0)),
 }
 
 

@@ -30,3 +30,32 @@ def test_golden_fixture_is_pinned_not_trivial():
         moved = max(np.max(np.abs(ref["p"] - case.fields["p"])),
                     np.max(np.abs(ref["ug"] - case.fields["ug"])))
         assert moved > 0.0, name
+
+
+EXPLICIT = sorted(n for n in CASES if n.startswith("explicit_"))
+
+
+def _explicit_run(name, iter_order=abi.ORDER_LEXICOGRAPHIC, stale_zero=1):
+    case, ref, _ = load(name)
+    s = cpu_solver(case, iter_order=iter_order)
+    s.lib.pdac_cpu_set_explicit_stale(s.ctx, stale_zero)
+    s.step(case.nsteps)
+    out = s.state()
+    return max(np.max(np.abs(out[k].reshape(v.shape) - v)) / max(np.max(np.abs(v)), 1e-300)
+               for k, v in ref.items())
+
+
+def test_explicit_fixtures_pin_the_cell_order():
+    """Reading the stage's starting state everywhere (no west/south/bottom
+    updates inside a stage) must miss the reference fixtures: they pin the
+    serial order of the stage loop."""
+    worst = max(_explicit_run(n, iter_order=abi.ORDER_STAGE) for n in EXPLICIT)
+    assert worst > 1e-8
+
+
+def test_explicit_fixtures_pin_the_unset_mass_flux_neighbours():
+    """Taking the reflected second neighbours where calc_all_fluxes leaves
+    them unset must miss the fixtures with MUSCL: they pin the zero the
+    reference build reads there."""
+    worst = max(_explicit_run(n, stale_zero=0) for n in EXPLICIT)
+    assert worst > 1e-8
