@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # order of enum pdac_kernel (include/pdacgpu.h)
 KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
            "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill",
-           "sweep_check", "iterate"]
+           "sweep_check", "iterate", "xinit", "xflux", "xrhs", "xchain", "xmass", "xsolve"]
 
 
 def words(D, S, G):
@@ -151,6 +151,63 @@ def run_form(args, precision, ids):
             "sweeps_prof": sweeps_prof, "prof": prof, "visits": visits.value, "final": final}
 
 
+def make_explicit_case(nx, nz):
+    """2D cylindrical eruption column on uniform 2 m cells advanced by the
+    explicit solver (time_integration = 1, rungekut = 1: one three-stage
+    Runge-Kutta update per step)."""
+    from pdac_amd.cases import eruption_column
+    case = eruption_column(nx=nx, nz=nz, dr0=2.0, dr1=2.0, nsteps=1, dt=1.0e-4)
+    case.time_integration = 1
+    case.rungekut = 1
+    return case
+
+
+def run_explicit(args, precision, order, ids):
+    """Warm-up and timed region of the explicit step on a fresh copy of its
+    workload, then a second region with HIP events around every kernel."""
+    from pdac_amd.gpu import gpu_solver
+    case = make_explicit_case(args.explicit_nx, args.explicit_nz)
+    g = gpu_solver(case, device=0, precision=precision, iter_order=order)
+    if args.warmup:
+        g.step(args.warmup)
+    g.lib.pdac_sync(g.ctx)
+    t0 = time.perf_counter()
+    g.step(args.explicit_steps)
+    g.lib.pdac_sync(g.ctx)
+    elapsed = time.perf_counter() - t0
+    g.lib.pdac_profile_select(g.ctx, 0xFFFFFFFF)
+    g.lib.pdac_profile_enable(g.ctx, 1)
+    g.lib.pdac_profile_reset(g.ctx)
+    g.step(args.explicit_steps)
+    g.lib.pdac_sync(g.ctx)
+    prof = profile(g, ids)
+    g.lib.pdac_profile_enable(g.ctx, 0)
+    g.close()
+    return case, elapsed, prof
+
+
+def explicit_summary(args, ids):
+    from pdac_amd import abi
+    out = {}
+    runs = {"reference order": (abi.PREC_FP64, abi.ORDER_REDBLACK),
+            "stage order": (abi.PREC_FP64, abi.ORDER_STAGE),
+            "reference order FP32-flux": (abi.PREC_FP32FLUX, abi.ORDER_REDBLACK)}
+    for label, (prec, order) in runs.items():
+        case, elapsed, prof = run_explicit(args, prec, order, ids)
+        n = case.ntot
+        k = args.explicit_steps
+        xk = {name: {"launches": prof[name][0], "ms_per_step": round(prof[name][1] / k, 4)}
+              for name in KERNELS if prof[name][0]}
+        out[label] = {"value": round(n * k / elapsed / 1e6, 3),
+                      "ms_per_step": round(elapsed / k * 1e3, 4), "kernels": xk}
+        log(f"[explicit {label}] {k} steps: {elapsed:.3f} s")
+    return {"metric": "Mcell-steps/s",
+            "workload": f"2D cylindrical eruption column {args.explicit_nx}x{args.explicit_nz}, "
+                        f"uniform 2 m cells, 2 particle classes, explicit three-stage "
+                        f"Runge-Kutta (time_integration = 1)",
+            "ntot": case.ntot, "steps": args.explicit_steps, "dt": case.dt, "forms": out}
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64).reshape(-1)
     b = np.asarray(b, dtype=np.float64).reshape(-1)
@@ -171,6 +228,10 @@ def main():
                     help="precision forms to run; with both, value is the FP32-flux form's")
     ap.add_argument("--cpu-steps", type=int, default=16,
                     help="steps of the CPU sample (0: skip the CPU baseline)")
+    ap.add_argument("--explicit-steps", type=int, default=100,
+                    help="steps of the explicit-solver workload (0: skip it)")
+    ap.add_argument("--explicit-nx", type=int, default=256)
+    ap.add_argument("--explicit-nz", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the whole domain lives on one MI355X "
@@ -262,6 +323,8 @@ def main():
             "after_steps": args.warmup + 2 * args.steps,
             "p": rel_l2(a["final"]["p"], b["final"]["p"]),
             "eps_s": [rel_l2(a["final"]["eps_s"][s_], b["final"]["eps_s"][s_]) for s_ in range(S)]}
+    if args.explicit_steps > 0:
+        out["explicit"] = explicit_summary(args, ids)
     if args.cpu_steps > 0:
         t, nits = cpu_sample(make_case(args.nx, args.nz), args.cpu_steps)
         out["cpu_baseline"] = {

@@ -171,6 +171,30 @@ int pdac_step(pdac_ctx *ctx, double dt, int32_t rungekut, int32_t nsteps,
 int pdac_sync(pdac_ctx *ctx);
 const char *pdac_last_error(void);
 
+/* Ensembles of independent explicit-step domains of uneven size (no
+ * counterpart in the reference, which runs one domain per job).  The members
+ * are contexts set up as for pdac_step with time_integration = 1 and the
+ * same job_type, nsolid, ngas, precision and device; they keep their own
+ * device arrays and stay usable one by one.  pdac_ens_step advances every
+ * member nsteps steps of main_prog (boundary, initialize_fieldn, per stage
+ * explicit_iter and ygas) with its own dt[m]: one batched launch per kernel
+ * covers all members, and a step's launches are captured in a HIP graph on
+ * first use and replayed.  eos_fail[m] receives the largest EOS failure
+ * count of member m's explicit_iter calls, cfl[m] its last Courant number;
+ * PDAC_ST_EOS_FAILURE is returned after the batch when a member exceeded 1.
+ * pdac_ens_tune times each kernel class over the tile shapes 256x1, 128x2,
+ * 64x4, 32x8 (cells along i x rows) on the members' data and keeps the
+ * fastest (tiles[8] receives the choice per class: boundary, xinit, xflux,
+ * xrhs, xmass, xsolve, ygas flux, ygas update; us[8*4] the mean times); it
+ * leaves every state field unchanged.  Members must outlive the ensemble. */
+typedef struct pdac_ens pdac_ens;
+int pdac_ens_create(pdac_ens **ens, pdac_ctx *const *members, int count);
+int pdac_ens_destroy(pdac_ens *ens);
+int pdac_ens_tune(pdac_ens *ens, int32_t reps, int32_t *tiles, double *us);
+int pdac_ens_set_tiles(pdac_ens *ens, const int32_t *tiles);
+int pdac_ens_step(pdac_ens *ens, const double *dt, int32_t rungekut, int32_t nsteps,
+                  int32_t *eos_fail, double *cfl);
+
 /* Diagnostic: momentum fluxes of the last pdac_tilde call (compute_all_
  * fluxes_tilde, tilde.f:850-1326).  comp 0..2 = u,v,w; face 0..2 = e,n,t;
  * out receives ntot*(nsolid+1) values, phase-major. */

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpdacgpu.so")
 SOURCES = ["pdac_kernels.hip"]
 DEPS = ["pdac_kernels.hip", "pdac_iter.hip.inc", "pdac_energy.hip.inc", "pdac_explicit.hip.inc",
-        "pdac_api.hip.inc",
+        "pdac_ensemble.hip.inc", "pdac_api.hip.inc",
         "pdac_dev.hpp", "pdac_flux.hpp"]
 ARCH = os.environ.get("PDAC_OFFLOAD_ARCH", "gfx950")
 

@@ -174,9 +174,7 @@ __device__ __forceinline__ double extrapolate(double f2, double f1, double d12, 
 }
 
 template <int D, int NS>
-__global__ void __launch_bounds__(kBlock) k_boundary(Grid g, Consts K, Fields F) {
-  long c = cell_id();
-  if (c >= g.n) return;
+__device__ __forceinline__ void boundary_cell(const Grid &g, const Consts &K, Fields &F, long c) {
   const int32_t *fl = g.flag;
   int f = fl[c];
   if (!(f == FLUID || f == DOME_CELL || f == BL_CELL)) return;
@@ -378,6 +376,12 @@ __global__ void __launch_bounds__(kBlock) k_boundary(Grid g, Consts K, Fields F)
     }
   }
 #undef SETS
+}
+
+template <int D, int NS>
+__global__ void __launch_bounds__(kBlock) k_boundary(Grid g, Consts K, Fields F) {
+  const long c = cell_id();
+  if (c < g.n) boundary_cell<D, NS>(g, K, F, c);
 }
 
 // ======================================================================
@@ -779,4 +783,5 @@ __global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, do
 #include "pdac_iter.hip.inc"
 #include "pdac_energy.hip.inc"
 #include "pdac_explicit.hip.inc"
+#include "pdac_ensemble.hip.inc"
 #include "pdac_api.hip.inc"

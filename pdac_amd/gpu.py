@@ -24,6 +24,7 @@ EXPORTS = [
     "pdac_begin_step", "pdac_step", "pdac_sync", "pdac_last_error", "pdac_get_momflux",
     "pdac_profile_enable", "pdac_profile_select", "pdac_profile_reset", "pdac_profile_read",
     "pdac_profile_visits", "pdac_initialize_fieldn", "pdac_explicit_iter",
+    "pdac_ens_create", "pdac_ens_destroy", "pdac_ens_tune", "pdac_ens_set_tiles", "pdac_ens_step",
 ]
 
 
@@ -45,6 +46,13 @@ def lib() -> ct.CDLL:
                                         ct.POINTER(ct.c_double)]
         L.pdac_profile_visits.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int64)]
         L.pdac_sync.argtypes = [ct.c_void_p]
+        L.pdac_ens_create.argtypes = [ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_void_p), ct.c_int]
+        L.pdac_ens_destroy.argtypes = [ct.c_void_p]
+        L.pdac_ens_tune.argtypes = [ct.c_void_p, ct.c_int32, ct.POINTER(ct.c_int32),
+                                    ct.POINTER(ct.c_double)]
+        L.pdac_ens_set_tiles.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32)]
+        L.pdac_ens_step.argtypes = [ct.c_void_p, ct.POINTER(ct.c_double), ct.c_int32, ct.c_int32,
+                                    ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double)]
         _lib = L
     return _lib
 
@@ -66,3 +74,65 @@ def gpu_solver(case, device: int = 0, iter_order=abi.ORDER_REDBLACK, **kw) -> Gp
     L = lib()
     create = lambda ctx, dims: L.pdac_create(ctx, dims, device)  # noqa: E731
     return GpuSolver(L, "pdac_", create, case, iter_order=iter_order, **kw)
+
+
+TILE_SHAPES = [(256, 1), (128, 2), (64, 4), (32, 8)]
+TILE_KERNELS = ["boundary", "xinit", "xflux", "xrhs", "xmass", "xsolve", "ygas_flux", "ygas_update"]
+
+
+class Ensemble:
+    """Explicit-step domains advanced together by batched launches
+    (``pdac_ens_*``).  ``members`` are :class:`GpuSolver` objects of cases
+    with ``time_integration = 1`` that share dimension, particle classes,
+    gas species and precision."""
+
+    def __init__(self, members):
+        self.lib = lib()
+        self.members = list(members)
+        arr = (ct.c_void_p * len(self.members))(*[m.ctx.value for m in self.members])
+        self.ens = ct.c_void_p()
+        rc = self.lib.pdac_ens_create(ct.byref(self.ens), arr, len(self.members))
+        if rc != 0:
+            raise RuntimeError(f"pdac_ens_create returned {rc}: {self.lib.pdac_last_error().decode()}")
+
+    def _check(self, where, rc):
+        if rc != 0:
+            raise RuntimeError(f"{where} returned {rc}: {self.lib.pdac_last_error().decode()}")
+
+    def tune(self, reps=5):
+        """Choose a tile shape per kernel class; returns ({kernel: (tx, tz)},
+        {kernel: [mean us per tile shape]})."""
+        tiles = (ct.c_int32 * len(TILE_KERNELS))()
+        us = (ct.c_double * (len(TILE_KERNELS) * len(TILE_SHAPES)))()
+        self._check("pdac_ens_tune", self.lib.pdac_ens_tune(self.ens, reps, tiles, us))
+        chosen = {k: TILE_SHAPES[tiles[i]] for i, k in enumerate(TILE_KERNELS)}
+        times = {k: [us[i * len(TILE_SHAPES) + t] for t in range(len(TILE_SHAPES))]
+                 for i, k in enumerate(TILE_KERNELS)}
+        return chosen, times
+
+    def set_tiles(self, idx):
+        arr = (ct.c_int32 * len(TILE_KERNELS))(*idx)
+        self._check("pdac_ens_set_tiles", self.lib.pdac_ens_set_tiles(self.ens, arr))
+
+    def step(self, nsteps=1, dt=None, rungekut=1):
+        """Advance every member; returns (eos failures, Courant numbers) per member."""
+        n = len(self.members)
+        dts = (ct.c_double * n)(*[m.case.dt if dt is None else dt[i] for i, m in enumerate(self.members)])
+        fail = (ct.c_int32 * n)()
+        cfl = (ct.c_double * n)()
+        self._check("pdac_ens_step", self.lib.pdac_ens_step(self.ens, dts, rungekut, nsteps, fail, cfl))
+        for i, m in enumerate(self.members):
+            for _ in range(nsteps):
+                m.time = m.time + dts[i]
+        return list(fail), list(cfl)
+
+    def close(self):
+        if self.ens:
+            self.lib.pdac_ens_destroy(self.ens)
+            self.ens = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -126,7 +126,7 @@ def test_stage_order_matches_checker(name):
 def test_eruption_column_20_steps():
     from pdac_amd.cases import eruption_column
     print("[explicit eruption 48x96]", flush=True)
-    case = eruption_column(nx=48, nz=96, nsteps=20, dt=2.0e-4, vent_radius=10.0)
+    case = eruption_column(nx=48, nz=96, nsteps=20, dt=5.0e-5, vent_radius=10.0)
     case.time_integration = 1
     g, c = gpu(case), cpu_solver(case, iter_order=abi.ORDER_LEXICOGRAPHIC)
     g.step(case.nsteps)
@@ -153,7 +153,7 @@ def test_fp32flux_form_error():
     the FP64 form, every field within 1e-3."""
     from pdac_amd.cases import eruption_column
     print("[explicit fp32flux]", flush=True)
-    case = eruption_column(nx=48, nz=96, nsteps=50, dt=2.0e-4, vent_radius=10.0)
+    case = eruption_column(nx=48, nz=96, nsteps=50, dt=5.0e-5, vent_radius=10.0)
     case.time_integration = 1
     out = {}
     for prec in (abi.PREC_FP32FLUX, abi.PREC_FP64):
