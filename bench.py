@@ -208,6 +208,51 @@ def explicit_summary(args, ids):
             "ntot": case.ntot, "steps": args.explicit_steps, "dt": case.dt, "forms": out}
 
 
+def ensemble_summary(args):
+    """The hazard ensemble: members of uneven size (nx drawn from [128, 512],
+    nz = 2 nx, uniform 2 m cells), vent w, T and particle fraction drawn per
+    member, explicit step in the FP32-flux form, all advanced by batched
+    graph-replayed launches after the tile autotuner."""
+    from pdac_amd import abi
+    from pdac_amd.cases import eruption_column
+    from pdac_amd.gpu import Ensemble, gpu_solver
+    rng = np.random.default_rng(args.ensemble_seed)
+    t0 = time.perf_counter()
+    members = []
+    for m in range(args.ensemble_members):
+        nx = int(rng.integers(128, 513))
+        case = eruption_column(nx=nx, nz=2 * nx, dr0=2.0, dr1=2.0, nsteps=1, dt=1.0e-4,
+                               w_vent=float(rng.uniform(50.0, 200.0)),
+                               t_vent=float(rng.uniform(800.0, 1200.0)),
+                               eps_vent=float(rng.uniform(0.005, 0.03)), seed=1000 + m)
+        case.time_integration = 1
+        members.append(gpu_solver(case, device=0, precision=abi.PREC_FP32FLUX))
+    ncells = sum(m.case.ntot for m in members)
+    log(f"[ensemble] {len(members)} members, {ncells} cells, set up in {time.perf_counter() - t0:.1f} s")
+    e = Ensemble(members)
+    chosen, times = e.tune(reps=3)
+    log(f"[ensemble] tiles {chosen}")
+    e.step(args.warmup)
+    t1 = time.perf_counter()
+    fails, cfl = e.step(args.ensemble_steps)
+    elapsed = time.perf_counter() - t1
+    log(f"[ensemble] {args.ensemble_steps} steps: {elapsed:.3f} s, max cfl {max(cfl):.3f}")
+    e.close()
+    for m in members:
+        m.close()
+    return {"metric": "Mcell-steps/s",
+            "value": round(ncells * args.ensemble_steps / elapsed / 1e6, 3),
+            "ms_per_step": round(elapsed / args.ensemble_steps * 1e3, 4),
+            "workload": f"{args.ensemble_members} 2D cylindrical eruption columns, nx drawn from "
+                        f"[128, 512] (seed {args.ensemble_seed}), nz = 2 nx, uniform 2 m cells, "
+                        f"vent w U[50,200] m/s, T U[800,1200] K, particle fraction U[0.005,0.03], "
+                        f"2 classes, explicit Runge-Kutta, FP32-flux form, dt 1e-4 s",
+            "ntot": ncells, "steps": args.ensemble_steps,
+            "tiles": {k: f"{v[0]}x{v[1]}" for k, v in chosen.items()},
+            "tuner_us": {k: [round(x, 2) for x in v] for k, v in times.items() if max(v) > 0},
+            "max_eos_failures": max(fails), "max_cfl": round(max(cfl), 4)}
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64).reshape(-1)
     b = np.asarray(b, dtype=np.float64).reshape(-1)
@@ -232,6 +277,10 @@ def main():
                     help="steps of the explicit-solver workload (0: skip it)")
     ap.add_argument("--explicit-nx", type=int, default=256)
     ap.add_argument("--explicit-nz", type=int, default=512)
+    ap.add_argument("--ensemble-members", type=int, default=64,
+                    help="members of the hazard ensemble (0: skip it)")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-seed", type=int, default=2024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the whole domain lives on one MI355X "
@@ -325,6 +374,8 @@ def main():
             "eps_s": [rel_l2(a["final"]["eps_s"][s_], b["final"]["eps_s"][s_]) for s_ in range(S)]}
     if args.explicit_steps > 0:
         out["explicit"] = explicit_summary(args, ids)
+    if args.ensemble_members > 0:
+        out["ensemble"] = ensemble_summary(args)
     if args.cpu_steps > 0:
         t, nits = cpu_sample(make_case(args.nx, args.nz), args.cpu_steps)
         out["cpu_baseline"] = {

@@ -81,7 +81,7 @@ __device__ __forceinline__ void wl_append(int32_t *cnt, int32_t *list, bool pred
 // device counters: sweep state of iter, work-list lengths per colour, EOS
 // failures; the persistent iteration kernel uses three pairs of work-list
 // lengths (C_WLP), an arrival counter for its grid barriers and an error flag
-enum Ctr { C_DONE = 0, C_NIT = 2, C_EOS = 4, C_WL0 = 5, C_WL1 = 6, C_WLP = 8, C_BAR = 14,
+enum Ctr { C_DONE = 0, C_NIT = 2, C_PHB = 3, C_EOS = 4, C_WL0 = 5, C_WL1 = 6, C_WLP = 8, C_BAR = 14,
            C_ERR = 15, C_N = 16 };
 
 }  // namespace

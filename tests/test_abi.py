@@ -61,3 +61,11 @@ def test_create_rejects_bad_dims(lib, dims, msg):
     assert rc == abi.ST_BAD_ARGUMENT
     assert msg in lib.pdac_last_error().decode()
     assert not ctx.value
+
+
+def test_ensemble_rejects_empty_member_list(lib):
+    ens = ct.c_void_p()
+    rc = lib.pdac_ens_create(ct.byref(ens), ct.POINTER(ct.c_void_p)(), 0)
+    assert rc == abi.ST_BAD_ARGUMENT
+    assert "ensemble" in lib.pdac_last_error().decode()
+    assert not ens.value

@@ -31,7 +31,8 @@ CONVERGING = [c for c in ALL if c != "mixed_2d_limiters"]
 
 def gpu(case, mode="persistent"):
     """GPU solver; mode selects how the red-black sweeps are launched: all in
-    one persistent kernel, or two launches per colour phase."""
+    one persistent kernel, two launches per colour phase, or batches of such
+    launches replayed as HIP graphs."""
     import os
 
     from pdac_amd.gpu import gpu_solver
@@ -73,7 +74,7 @@ STAGES = [
 ]
 
 
-@pytest.mark.parametrize("mode", ["persistent", "launches"])
+@pytest.mark.parametrize("mode", ["persistent", "launches", "graph"])
 @pytest.mark.parametrize("name", ALL)
 def test_stagewise(name, mode):
     print(f"[stagewise] {name} {mode}", flush=True)
@@ -121,7 +122,7 @@ def test_momentum_fluxes(name):
             assert rel(got, want) <= 1e-14, (comp, face, rel(got, want))
 
 
-@pytest.mark.parametrize("mode", ["persistent", "launches"])
+@pytest.mark.parametrize("mode", ["persistent", "launches", "graph"])
 @pytest.mark.parametrize("name", ALL)
 def test_steps_match_checker(name, mode):
     print(f"[steps] {name} {mode}", flush=True)
