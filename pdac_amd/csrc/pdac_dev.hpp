@@ -260,6 +260,26 @@ __device__ __forceinline__ int caloric_eosg(const Consts &k, double sieg, const 
   return info;
 }
 
+// the same with the species count fixed at compile time, mass fractions in
+// registers
+template <int NG>
+__device__ __forceinline__ int caloric_eosg_n(const Consts &k, double sieg, const double (&yg)[NG],
+                                              double &tg, double &cg) {
+  int info = 1;
+  double cgas = 0.0;
+  for (int ii = 1; ii <= 2000; ++ii) {
+    double tgnn = tg;
+    cgas = 0.0;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) cgas = hcapg1(k, k.gas_type[g] - 1, tg) * yg[g] + cgas;
+    tg = 0.0 + (sieg - 0.0) / cgas;
+    if (fabs((tgnn - tg) / tgnn) <= 1.0e-8) { info = 0; break; }
+  }
+  cg = cgas;
+  if (tg <= 0.0) info = 1;
+  return info;
+}
+
 // Lennard-Jones / Wilke mixture viscosity and conductivity, visc.f:37-148
 __device__ __forceinline__ void viscon(const Consts &k, const double *xgc, double tg,
                                        double &mu, double &kap) {
