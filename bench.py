@@ -131,6 +131,9 @@ def run_form(args, precision, ids):
         mask = 0xFFFFFFFF if args.kernel_timing == "all" else (1 << ids["sweep"]) | (1 << ids["iterate"])
         g.lib.pdac_profile_select(g.ctx, mask)
         g.lib.pdac_profile_enable(g.ctx, 1)
+    visits0 = ct.c_int64()
+    if g.lib.pdac_profile_visits(g.ctx, ct.byref(visits0)) != 0:
+        raise RuntimeError(g.last_error())
     g.lib.pdac_profile_reset(g.ctx)
     nit_before = g.nit_total
     t1 = time.perf_counter()
@@ -144,6 +147,7 @@ def run_form(args, precision, ids):
         raise RuntimeError(g.last_error())
     g.lib.pdac_profile_enable(g.ctx, 0)
     log(f"[{tag}] event-timed {args.steps} steps: {elapsed_prof:.3f} s, {sweeps_prof} sweeps")
+    log(f"[{tag}] sweep work-list cells over the whole run: {visits0.value + visits.value}")
     rho = np.asarray(case.density[:case.nsolid], dtype=np.float64).reshape(-1, 1)
     final = {"p": g.get("p"), "eps_s": g.get("rlk").reshape(case.nsolid, -1) / rho}
     g.close()
