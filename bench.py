@@ -35,6 +35,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# HBM bytes moved per work-list cell by the sweep kernel (k_sweep_active),
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE over every dispatch of the default
+# workload (FP32-flux form), each scaled by what the counter reads for a copy
+# of known size (--copy-calibration), divided by the work-list cells of the
+# same run
+SWEEP_TRAFFIC_PER_CELL = 2038.0
 # order of enum pdac_kernel (include/pdacgpu.h)
 KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
            "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill",
@@ -257,6 +263,20 @@ def ensemble_summary(args):
             "max_eos_failures": max(fails), "max_cfl": round(max(cfl), 4)}
 
 
+def copy_calibration():
+    """A plain device copy of known size (3 x 64 MiB) for scaling the
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = 8 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64).reshape(-1)
     b = np.asarray(b, dtype=np.float64).reshape(-1)
@@ -285,11 +305,16 @@ def main():
                     help="members of the hazard ensemble (0: skip it)")
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--ensemble-seed", type=int, default=2024)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 3 x 64 MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the whole domain lives on one MI355X "
               f"(--gpus {args.gpus} requested).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     from pdac_amd import abi
 
     ids = {name: i for i, name in enumerate(KERNELS)}
@@ -312,10 +337,13 @@ def main():
     if launches == 0:
         dom, ms = "none", 0.0
     per_launch_s = ms / 1e3 / max(launches, 1) if launches else float("nan")
+    traffic = None
     if dom == "sweep":
         # one launch runs the in-cell iteration of the cells of one colour that
         # were not converged at visit start (the work list)
         bytes_launch = 8.0 * w["Is"] * visits / max(launches, 1)
+        if SWEEP_TRAFFIC_PER_CELL is not None:
+            traffic = round(SWEEP_TRAFFIC_PER_CELL * visits / max(launches, 1))
         unit_note = (f"cells per launch = mean work-list length {visits / max(launches, 1):.0f} "
                      f"(cells of one colour not converged at visit start), {w['Is']} words each")
     elif dom == "iterate":
@@ -357,7 +385,7 @@ def main():
                    "iter_order": "red-black"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": bytes_launch, "mean_launch_us": round(per_launch_s * 1e6, 2),
                      "launches": launches, "units": unit_note},
         "sweep_visits_per_step": round(visits / args.steps, 1),
