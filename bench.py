@@ -35,12 +35,11 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# HBM bytes moved per work-list cell by the sweep kernel (k_sweep_active),
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE over every dispatch of the default
+# HBM bytes per launch of the sweep kernel (k_sweep, one colour phase), from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE over every dispatch of the default
 # workload (FP32-flux form), each scaled by what the counter reads for a copy
-# of known size (--copy-calibration), divided by the work-list cells of the
-# same run
-SWEEP_TRAFFIC_PER_CELL = 2038.0
+# of known size (--copy-calibration); None until measured for this kernel
+SWEEP_TRAFFIC_PER_LAUNCH = None
 # order of enum pdac_kernel (include/pdacgpu.h)
 KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
            "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill",
@@ -338,14 +337,22 @@ def main():
         dom, ms = "none", 0.0
     per_launch_s = ms / 1e3 / max(launches, 1) if launches else float("nan")
     traffic = None
+    worklist = None
     if dom == "sweep":
-        # one launch runs the in-cell iteration of the cells of one colour that
-        # were not converged at visit start (the work list)
-        bytes_launch = 8.0 * w["Is"] * visits / max(launches, 1)
-        if SWEEP_TRAFFIC_PER_CELL is not None:
-            traffic = round(SWEEP_TRAFFIC_PER_CELL * visits / max(launches, 1))
-        unit_note = (f"cells per launch = mean work-list length {visits / max(launches, 1):.0f} "
-                     f"(cells of one colour not converged at visit start), {w['Is']} words each")
+        # one launch is one colour phase of a sweep: the check of every cell of
+        # the colour and the in-cell iteration of those not converged at visit
+        # start.  Algorithmic bytes follow the reference's sweep, which reads
+        # and writes the 62 sweep words of every cell; the kernel skips cells
+        # whose inputs did not change since their last check (same result)
+        cells = n / 2.0
+        bytes_launch = 8.0 * w["Is"] * cells
+        if SWEEP_TRAFFIC_PER_LAUNCH is not None:
+            traffic = round(SWEEP_TRAFFIC_PER_LAUNCH)
+        unit_note = (f"cells per launch = ntot/2 = {cells:.0f} (one colour phase of a sweep), "
+                     f"{w['Is']} words each")
+        wl = visits / max(launches, 1)
+        worklist = {"cells_per_launch": round(wl, 1), "bytes_per_launch": 8.0 * w["Is"] * wl,
+                    "gbs": round(8.0 * w["Is"] * wl / per_launch_s / 1e9, 2)}
     elif dom == "iterate":
         # one launch runs every sweep of one iter call
         bytes_launch = 8.0 * w["Is"] * n * sweeps_prof / max(launches, 1)
@@ -387,7 +394,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": bytes_launch, "mean_launch_us": round(per_launch_s * 1e6, 2),
-                     "launches": launches, "units": unit_note},
+                     "launches": launches, "units": unit_note,
+                     "worklist": worklist},
         "sweep_visits_per_step": round(visits / args.steps, 1),
         "event_region": {"steps": args.steps, "seconds": round(r["elapsed_prof"], 4),
                          "sweeps_per_step": round(sweeps_prof / args.steps, 3),

@@ -78,10 +78,12 @@ __device__ __forceinline__ void wl_append(int32_t *cnt, int32_t *list, bool pred
   if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = val;
 }
 
-// device counters: sweep state of iter, work-list lengths per colour, EOS
-// failures; the persistent iteration kernel uses three pairs of work-list
-// lengths (C_WLP), an arrival counter for its grid barriers and an error flag
-enum Ctr { C_DONE = 0, C_NIT = 2, C_PHB = 3, C_EOS = 4, C_WL0 = 5, C_WL1 = 6, C_WLP = 8, C_BAR = 14,
+// device counters: sweep state of iter (done flag, sweep numbers of the two
+// colour launches, the sweep that converged, the first colour phase of the
+// call), EOS failures, three pairs of unconverged-cell counts per sweep
+// (C_WLP), and for the persistent iteration kernel an arrival counter for its
+// grid barriers and an error flag
+enum Ctr { C_DONE = 0, C_NIT1 = 1, C_NIT = 2, C_PHB = 3, C_EOS = 4, C_NITF = 5, C_WLP = 8, C_BAR = 14,
            C_ERR = 15, C_N = 16 };
 
 }  // namespace
