@@ -174,6 +174,58 @@ def eruption_column(nx=512, nz=1024, nsolid=2, seed=1234, dr0=1.0, dr1=20.0,
     return fill_state(case, p, tg, ts, eps, ygc, wg=wg, ws=ws)
 
 
+def summit_vent_3d(nx=256, ny=256, nz=192, d=10.0, nsolid=3, seed=1234, vent_half=50.0,
+                   w_vent=100.0, t_vent=1000.0, eps_vent=0.02, nsteps=1000, dt=None,
+                   **kw) -> Case:
+    """3D Cartesian eruption from a square vent on flat ground.
+
+    Uniform ``d`` cells; particle classes 30, 250 and 1000 um with densities
+    2700, 2000 and 1000 kg/m3; air + water vapour; stratified standard
+    atmosphere.  The vent is the square of half-width ``vent_half`` at the
+    centre of the lowest fluid layer: gas w = ``w_vent`` with seeded 2 %
+    noise, T = ``t_vent``, particle fraction ``eps_vent`` split evenly, water
+    vapour only.  The ground is a no-slip wall, the four sides free
+    in/outflow, the top a slip lid.  (This is the shape of the 3D cone case
+    with the topography left out: immersed boundaries are not part of this
+    library.)"""
+    if dt is None:
+        dt = 0.2 * d / (w_vent + 350.0)
+    diam = [30.0, 250.0, 1000.0][:nsolid] + [100.0] * max(0, nsolid - 3)
+    dens = [2700.0, 2000.0, 1000.0][:nsolid] + [2500.0] * max(0, nsolid - 3)
+    case = Case(job_type="3D", itc=0, nx=nx, ny=ny, nz=nz, dx=np.full(nx, d), dy=np.full(ny, d),
+                dz=np.full(nz, d), west=4, east=4, south=4, north=4, bottom=3, top=2,
+                nsolid=nsolid, diameter=diam, density=dens, sphericity=[1.0] * nsolid,
+                viscosity=[0.5] * nsolid, specific_heat=[1200.0] * nsolid,
+                thermal_conductivity=[2.0] * nsolid, twophase_limit=[1.0e-8] * nsolid,
+                gravz=-9.81, stratification=True, dt=dt, nsteps=nsteps, **kw)
+    from .atmosphere import atmosphere_profile
+    p_atm, t_atm = atmosphere_profile(case)
+    rng = np.random.default_rng(seed)
+    n = case.ntot
+    kk = np.repeat(np.arange(nz), nx * ny)
+    jj = np.tile(np.repeat(np.arange(ny), nx), nz)
+    ii = np.tile(np.arange(nx), ny * nz)
+    xc = (np.arange(nx) - 0.5 * (nx - 1)) * d
+    yc = (np.arange(ny) - 0.5 * (ny - 1)) * d
+    p = p_atm[kk].copy()
+    tg = t_atm[kk].copy()
+    eps = np.zeros((nsolid, n))
+    ygc = np.stack([np.zeros(n), np.ones(n)])
+    wg = np.zeros(n)
+    ws = np.zeros((nsolid, n))
+    vent = (kk == 1) & (np.abs(xc[ii]) <= vent_half) & (np.abs(yc[jj]) <= vent_half)
+    noise = rng.uniform(-1.0, 1.0, size=n)
+    wg[vent] = w_vent * (1.0 + 0.02 * noise[vent])
+    for s_ in range(nsolid):
+        ws[s_][vent] = wg[vent]
+        eps[s_][vent] = eps_vent / nsolid
+    tg[vent] = t_vent
+    ygc[0][vent] = 1.0
+    ygc[1][vent] = 0.0
+    ts = np.broadcast_to(tg, (nsolid, n)).copy()
+    return fill_state(case, p, tg, ts, eps, ygc, wg=wg, ws=ws)
+
+
 def mixed_box(three_d=False, nx=10, ny=7, nz=9, nsolid=2, seed=7, itc=0,
               gravz=-9.81, nsteps=2, dt=2.0e-4, **kw) -> Case:
     """Small randomised domain for parity tests: non-uniform cells, several
