@@ -35,6 +35,7 @@ struct Grid {
   const double *indx, *indy, *indz;
   const double *r, *rb, *inr, *inrb;
   const int32_t *flag;
+  const uint8_t *reg;                  // 1: the cell's full stencil needs no wall reflection
   const double *p_atm, *t_atm;         // 1..nz
   int nx, ny, nz;
   long n;

@@ -35,10 +35,43 @@ struct FullIdx {
   long Reee, Rnnn, Rttt, Reen, Reet, Renn, Rnnt, Rett, Rntt;
 };
 
+// the stencil of a cell marked regular (k_regular): every index is the cell
+// index plus a fixed offset, and every reflected index is the raw one
+template <int D>
+__device__ __forceinline__ void full_idx_off(const Grid &g, FullIdx<D> &x) {
+  const long c = x.c;
+  const long sy = D == 3 ? (long)g.nx : 0, sz = (long)g.nx * g.ny;
+  x.e = c + 1; x.w = c - 1; x.t = c + sz; x.b = c - sz;
+  x.ee = c + 2; x.tt = c + 2 * sz;
+  x.et = c + 1 + sz; x.eb = c + 1 - sz; x.wt = c - 1 + sz;
+  x.eee = c + 3; x.ttt = c + 3 * sz;
+  x.eet = c + 2 + sz; x.ett = c + 1 + 2 * sz;
+  if (D == 2) {
+    x.n = x.s = x.nn = x.en = x.es = x.nt = x.wn = x.nb = x.st = c;
+    x.nnn = x.een = x.enn = x.nnt = x.ntt = c;
+  } else {
+    x.n = c + sy; x.s = c - sy; x.nn = c + 2 * sy;
+    x.en = c + 1 + sy; x.es = c + 1 - sy; x.nt = c + sy + sz; x.wn = c - 1 + sy;
+    x.nb = c + sy - sz; x.st = c - sy + sz;
+    x.nnn = c + 3 * sy;
+    x.een = c + 2 + sy; x.enn = c + 1 + 2 * sy; x.nnt = c + 2 * sy + sz; x.ntt = c + sy + 2 * sz;
+  }
+  x.Re = x.e; x.Rw = x.w; x.Rn = x.n; x.Rs = x.s; x.Rt = x.t; x.Rb = x.b;
+  x.Ree = x.ee; x.Rnn = x.nn; x.Rtt = x.tt;
+  x.Ren = x.en; x.Ret = x.et; x.Res = x.es; x.Reb = x.eb; x.Rnt = x.nt; x.Rwn = x.wn;
+  x.Rnb = x.nb; x.Rwt = x.wt; x.Rst = x.st;
+  x.Reee = x.eee; x.Rnnn = x.nnn; x.Rttt = x.ttt; x.Reen = x.een; x.Reet = x.eet;
+  x.Renn = x.enn; x.Rnnt = x.nnt; x.Rett = x.ett; x.Rntt = x.ntt;
+}
+
 template <int D>
 __device__ __forceinline__ void full_idx(const Grid &g, int i, int j, int k, FullIdx<D> &x) {
   const int32_t *fl = g.flag;
   x.c = cell_index(g, i, j, k);
+  if (g.reg && g.reg[x.c]) {
+    full_idx_off<D>(g, x);
+    return;
+  }
   x.e = rawn(g, i, j, k, 1, 0, 0);  x.w = rawn(g, i, j, k, -1, 0, 0);
   x.t = rawn(g, i, j, k, 0, 0, 1);  x.b = rawn(g, i, j, k, 0, 0, -1);
   x.ee = rawn(g, i, j, k, 2, 0, 0); x.tt = rawn(g, i, j, k, 0, 0, 2);

@@ -51,6 +51,15 @@ inline unsigned nblocks(long n) {
 
 // decode a linear cell index into 1-based (i,j,k)
 __device__ __forceinline__ void decode(const Grid &g, long c, int &i, int &j, int &k) {
+  if (g.n <= 0x7fffffffL) {
+    // 32-bit quotients (a mesh of fewer than 2^31 cells)
+    const unsigned cu = (unsigned)c, nx = (unsigned)g.nx, nxy = nx * (unsigned)g.ny;
+    const unsigned kq = cu / nxy, r = cu - kq * nxy, jq = r / nx;
+    k = (int)kq + 1;
+    j = (int)jq + 1;
+    i = (int)(r - jq * nx) + 1;
+    return;
+  }
   long nxy = (long)g.nx * g.ny;
   k = (int)(c / nxy) + 1;
   long r = c - (long)(k - 1) * nxy;
@@ -460,6 +469,34 @@ __global__ void __launch_bounds__(kBlock) k_copy(double *dst, const double *src,
 // fluxes of one phase (blockIdx.y) of each computed cell.  E/N/T faces are
 // owned by the cell; a W/S/B face is written only when that neighbour is
 // not computed (then no other cell writes it).
+// 1 for a computed cell whose full momentum-flux stencil (full_idx) lies
+// inside the mesh and needs no wall reflection and no mesh-edge special
+// case, else 0; 'g' carries no mask, so full_idx takes its general path here
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_regular(Grid g, uint8_t *reg) {
+  const long c = cell_id();
+  if (c >= g.n) return;
+  uint8_t r = 0;
+  if (computed(g.flag[c])) {
+    int i, j, k;
+    decode(g, c, i, j, k);
+    FullIdx<D> x;
+    full_idx<D>(g, i, j, k, x);
+    // offsets -1..+3 along every axis stay inside the mesh, so each raw
+    // index is the cell index plus a fixed offset
+    bool same = i >= 2 && i + 3 <= g.nx && k >= 2 && k + 3 <= g.nz && (D == 2 || (j >= 2 && j + 3 <= g.ny));
+    same = same && x.Re == x.e && x.Rw == x.w && x.Rn == x.n && x.Rs == x.s && x.Rt == x.t && x.Rb == x.b;
+    same = same && x.Ree == x.ee && x.Rnn == x.nn && x.Rtt == x.tt;
+    same = same && x.Ren == x.en && x.Ret == x.et && x.Res == x.es && x.Reb == x.eb && x.Rnt == x.nt;
+    same = same && x.Rwn == x.wn && x.Rnb == x.nb && x.Rwt == x.wt && x.Rst == x.st;
+    same = same && x.Reee == x.eee && x.Rnnn == x.nnn && x.Rttt == x.ttt && x.Reen == x.een;
+    same = same && x.Reet == x.eet && x.Renn == x.enn && x.Rnnt == x.nnt && x.Rett == x.ett;
+    same = same && x.Rntt == x.ntt;
+    r = same ? 1 : 0;
+  }
+  reg[c] = r;
+}
+
 template <int D, typename FT>
 __device__ __forceinline__ void momflux_cell(const Grid &g, const Consts &K, const Fields &F, long c,
                                              int fc, int ph, const double *uu, const double *vv,
