@@ -605,10 +605,53 @@ __device__ __forceinline__ void inter_d(const Consts &K, const double *kpgv, dou
   }
 }
 
+// interphase matrix of one axis (inter, drag.f:173-246): packed symmetric
+// entries with zero row sums, from the drag coefficients, the
+// particle-particle factors con (packed by pair, ppdrag) and the relative
+// particle velocities along the axis.  The entries of each axis depend on
+// that axis only, so the three axes of inter_d come out of three calls.
+template <int NS>
+__device__ __forceinline__ void inter_axis(const double *kpgv, double dt, const double *con,
+                                           const double (&d_pp)[NS][NS], double *a) {
+  constexpr int NP = NS + 1;
+  int ks = 0, pc = 0;
+#pragma unroll
+  for (int k = 1; k <= NP; ++k) {
+    if (k == 1) {
+      ks = ks + 1;
+      a[ks - 1] = 0.0;
+    } else {
+#pragma unroll
+      for (int kk = 1; kk <= k; ++kk) {
+        ks = ks + 1;
+        if (kk == 1) {
+          a[ks - 1] = -kpgv[k - 2] * dt;
+        } else if (kk == k) {
+          a[ks - 1] = 0.0;
+        } else {
+          a[ks - 1] = -(con[pc] * d_pp[k - 2][kk - 2]) * dt;
+          ++pc;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 1; k <= NP; ++k) {
+    double sx = 0.0;
+#pragma unroll
+    for (int kk = 1; kk <= k; ++kk) sx = sx + a[kk + k * (k - 1) / 2 - 1];
+#pragma unroll
+    for (int kk = k; kk <= NP; ++kk) sx = sx + a[k + kk * (kk - 1) / 2 - 1];
+    a[k * (k + 1) / 2 - 1] = -sx;
+  }
+}
+
 // tilde (part 2), mesh_loop of tilde.f:418-806: explicit momenta, drag,
 // interphase matrix and the (1-alpha) coupling.  Neighbour interphase
 // entries come from the copy taken before this call (the serial loop reads
-// east/north/top neighbours before it reaches them).
+// east/north/top neighbours before it reaches them).  The drag coefficients
+// need every velocity component; the rest is done one axis at a time (the
+// axes share no accumulator), which keeps one axis's values live at once.
 template <int D, int NS, typename FT>
 __global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, double dt, int old_zero) {
   long c = cell_id();
@@ -616,166 +659,143 @@ __global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, do
   if (!computed(g.flag[c])) return;
   constexpr int NP = NS + 1;
   constexpr int NA = NP * (NP + 1) / 2;
+  constexpr int NPAIR = NS * (NS - 1) / 2;
   long n = g.n;
   int i, j, k;
   decode(g, c, i, j, k);
   Nbr<D> q;
   first_nbrs<D>(g, i, j, k, q);
-  // momentum fluxes (stored in FT); their differences are taken in double
-  const FT *mf[3][3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int b = 0; b < 3; ++b) mf[a][b] = (const FT *)F.mf[a][b];
   const double *dx = g.dx, *dy = g.dy, *dz = g.dz;
   const double *rgpn = F.f[PDAC_RGPN], *rlkn = F.f[PDAC_RLKN], *pn = F.f[PDAC_PN], *epn = F.f[PDAC_EPN];
   const double *ug = F.f[PDAC_UG], *vg = F.f[PDAC_VG], *wg = F.f[PDAC_WG];
   const double *us = F.f[PDAC_US], *vs = F.f[PDAC_VS], *ws = F.f[PDAC_WS];
-  double ugfx = (double)mf[0][0][c] - (double)mf[0][0][q.imjk];
-  double ugfz = (double)mf[0][2][c] - (double)mf[0][2][q.ijkm];
-  double wgfx = (double)mf[2][0][c] - (double)mf[2][0][q.imjk];
-  double wgfz = (double)mf[2][2][c] - (double)mf[2][2][q.ijkm];
-  double ugfy = 0.0, vgfx = 0.0, vgfy = 0.0, vgfz = 0.0, wgfy = 0.0;
-  if (D == 3) {
-    ugfy = (double)mf[0][1][c] - (double)mf[0][1][q.ijmk];
-    vgfx = (double)mf[1][0][c] - (double)mf[1][0][q.imjk];
-    vgfy = (double)mf[1][1][c] - (double)mf[1][1][q.ijmk];
-    vgfz = (double)mf[1][2][c] - (double)mf[1][2][q.ijkm];
-    wgfy = (double)mf[2][1][c] - (double)mf[2][1][q.ijmk];
-  }
-  double dxi = dx[i], dxip1 = dx[i + 1], dyj = dy[j], dyjp1 = dy[j + 1];
-  double dzk = dz[k], dzkp1 = dz[k + 1];
-  double indxp = 1.0 / (dx[i] + dx[i + 1]), indyp = 1.0 / (dy[j] + dy[j + 1]);
-  double indzp = 1.0 / (dz[k] + dz[k + 1]);
-  const double alpha = K.alpha, ag = K.alphagrav;
-  double epn_e, epn_n, epn_t;
-  if (K.pmodel == 1) {
-    epn_e = (dxi * epn[q.e] + dxip1 * epn[c]) * indxp;
-    epn_n = (dyj * epn[q.n] + dyjp1 * epn[c]) * indyp;
-    epn_t = (dzk * epn[q.t] + dzkp1 * epn[c]) * indzp;
-  } else {
-    epn_e = epn_n = epn_t = 1.0;
-  }
-  double t, rug, rvg = 0.0, rwg;
-  t = 0.0;
-  t = t - indxp * 2.0 * ugfx * g.inrb[i];
-  t = t + (1.0 - ag) * (dx[i + 1] * rgpn[c] + dx[i] * rgpn[q.e]) * indxp * K.gravx;
-  t = t - g.indy[j] * ugfy;
-  t = t - g.indz[k] * ugfz;
-  t = t + (1.0 - alpha) * indxp * 2.0 * epn_e * (pn[c] - pn[q.e]);
-  rug = F.f[PDAC_RUGN][c] + dt * t;
-  if (D == 3) {
-    t = 0.0;
-    t = t + (1.0 - ag) * (dy[j + 1] * rgpn[c] + dy[j] * rgpn[q.n]) * indyp * K.gravy;
-    t = t - g.indx[i] * vgfx;
-    t = t - indyp * 2.0 * vgfy;
-    t = t - g.indz[k] * vgfz;
-    t = t + (1.0 - alpha) * indyp * 2.0 * epn_n * (pn[c] - pn[q.n]);
-    rvg = F.f[PDAC_RVGN][c] + dt * t;
-  }
-  t = 0.0;
-  t = t + (1.0 - ag) * (dz[k + 1] * rgpn[c] + dz[k] * rgpn[q.t]) * indzp * K.gravz;
-  t = t - g.indx[i] * wgfx * g.inr[i];
-  t = t - g.indy[j] * wgfy;
-  t = t - indzp * 2.0 * wgfz;
-  t = t + (1.0 - alpha) * indzp * 2.0 * epn_t * (pn[c] - pn[q.t]);
-  rwg = F.f[PDAC_RWGN][c] + dt * t;
-  double rus[NS], rvs[NS], rws[NS], kpgv[NS], rlkc[NS];
   const double *rgp = F.f[PDAC_RGP], *rlk = F.f[PDAC_RLK], *ep = F.f[PDAC_EP], *mug = F.f[PDAC_MUG];
+  const double alpha = K.alpha, ag = K.alphagrav;
+  // drag coefficients and particle-particle factors (drag.f:60-157, 265-295)
+  double kpgv[NS], rlkc[NS], con[NPAIR > 0 ? NPAIR : 1];
 #pragma unroll
   for (int is = 0; is < NS; ++is) {
-    long o = (long)(is + 1) * n, os = (long)is * n;
+    long os = (long)is * n;
     rlkc[is] = rlk[os + c];
-    double usfx = (double)mf[0][0][o + c] - (double)mf[0][0][o + q.imjk];
-    double usfz = (double)mf[0][2][o + c] - (double)mf[0][2][o + q.ijkm];
-    double wsfx = (double)mf[2][0][o + c] - (double)mf[2][0][o + q.imjk];
-    double wsfz = (double)mf[2][2][o + c] - (double)mf[2][2][o + q.ijkm];
-    double usfy = 0.0, vsfx = 0.0, vsfy = 0.0, vsfz = 0.0, wsfy = 0.0;
-    if (D == 3) {
-      usfy = (double)mf[0][1][o + c] - (double)mf[0][1][o + q.ijmk];
-      vsfx = (double)mf[1][0][o + c] - (double)mf[1][0][o + q.imjk];
-      vsfy = (double)mf[1][1][o + c] - (double)mf[1][1][o + q.ijmk];
-      vsfz = (double)mf[1][2][o + c] - (double)mf[1][2][o + q.ijkm];
-      wsfy = (double)mf[2][1][o + c] - (double)mf[2][1][o + q.ijmk];
-    }
-    double epsn_e, epsn_n, epsn_t;
-    if (K.pmodel == 1) {
-      epsn_e = (dxi * rlkn[os + q.e] + dxip1 * rlkn[os + c]) * indxp * K.inrl[is];
-      epsn_n = (dyj * rlkn[os + q.n] + dyjp1 * rlkn[os + c]) * indyp * K.inrl[is];
-      epsn_t = (dzk * rlkn[os + q.t] + dzkp1 * rlkn[os + c]) * indzp * K.inrl[is];
-    } else {
-      epsn_e = epsn_n = epsn_t = 0.0;
-    }
-    const double sink = 0.0;
-    t = 0.0;
-    t = t - indxp * 2.0 * usfx * g.inrb[i];
-    t = t + (1.0 - ag) * indxp * K.gravx * (dx[i + 1] * rlkn[os + c] + dx[i] * rlkn[os + q.e]);
-    t = t - g.indy[j] * usfy;
-    t = t - g.indz[k] * usfz;
-    t = t + (1.0 - alpha) * indxp * 2.0 * epsn_e * (pn[c] - pn[q.e]);
-    rus[is] = F.f[PDAC_RUSN][os + c] + dt * t;
-    rus[is] = rus[is] + dt * sink * us[os + c];
-    if (D == 3) {
-      t = 0.0;
-      t = t + (1.0 - ag) * indyp * K.gravy * (dy[j + 1] * rlkn[os + c] + dy[j] * rlkn[os + q.n]);
-      t = t - g.indx[i] * vsfx;
-      t = t - indyp * 2.0 * vsfy;
-      t = t - g.indz[k] * vsfz;
-      t = t + (1.0 - alpha) * indyp * 2.0 * epsn_n * (pn[c] - pn[q.n]);
-      rvs[is] = F.f[PDAC_RVSN][os + c] + dt * t;
-      rvs[is] = rvs[is] + dt * sink * vs[os + c];
-    } else {
-      rvs[is] = 0.0;
-    }
-    t = 0.0;
-    t = t + (1.0 - ag) * indzp * K.gravz * (rlkn[os + c] * dz[k + 1] + rlkn[os + q.t] * dz[k]);
-    t = t - g.indx[i] * wsfx * g.inr[i];
-    t = t - g.indy[j] * wsfy;
-    t = t - indzp * 2.0 * wsfz;
-    t = t + (1.0 - alpha) * indzp * 2.0 * epsn_t * (pn[c] - pn[q.t]);
-    rws[is] = F.f[PDAC_RWSN][os + c] + dt * t;
-    rws[is] = rws[is] + dt * sink * ws[os + c];
     double dugs = ((ug[c] - us[os + c]) + (ug[q.imjk] - us[os + q.imjk])) * 0.5;
     double dwgs = ((wg[c] - ws[os + c]) + (wg[q.ijkm] - ws[os + q.ijkm])) * 0.5;
     double dvgs = (D == 3) ? ((vg[c] - vs[os + c]) + (vg[q.ijmk] - vs[os + q.ijmk])) * 0.5 : 0.0;
     if (K.drag_model == 1) kpgv[is] = kdrags(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c]);
     else kpgv[is] = sdrag(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c]);
   }
-  // relative particle-particle velocities at the cell centre (drag.f:216-218)
-  double du_pp[NS][NS], dv_pp[NS][NS], dw_pp[NS][NS];
-#pragma unroll
-  for (int a = 0; a < NS; ++a)
-#pragma unroll
-    for (int b = 0; b < NS; ++b) {
-      long oa = (long)a * n, ob = (long)b * n;
-      du_pp[a][b] = fabs(us[oa + c] - us[ob + c] + us[oa + q.imjk] - us[ob + q.imjk]) * 0.5;
-      dw_pp[a][b] = fabs(ws[oa + c] - ws[ob + c] + ws[oa + q.ijkm] - ws[ob + q.ijkm]) * 0.5;
-      dv_pp[a][b] = (D == 3) ? fabs(vs[oa + c] - vs[ob + c] + vs[oa + q.ijmk] - vs[ob + q.ijmk]) * 0.5 : 0.0;
-    }
-  double au[NA], av[NA], aw[NA];
-  inter_d<NS>(K, kpgv, dt, rlkc, du_pp, dv_pp, dw_pp, au, av, aw);
-  double *appu = F.f[PDAC_APPU], *appv = F.f[PDAC_APPV], *appw = F.f[PDAC_APPW];
-#pragma unroll
-  for (int a = 0; a < NA; ++a) {
-    appu[(long)a * n + c] = au[a];
-    appw[(long)a * n + c] = aw[a];
-    if (D == 3) appv[(long)a * n + c] = av[a];
-  }
   {
+    int pc = 0;
+#pragma unroll
+    for (int kk0 = 3; kk0 <= NP; ++kk0)
+#pragma unroll
+      for (int kk = 2; kk < kk0; ++kk) con[pc++] = ppdrag(K, rlkc, kk0, kk);
+  }
+  double dxi = dx[i], dxip1 = dx[i + 1], dyj = dy[j], dyjp1 = dy[j + 1];
+  double dzk = dz[k], dzkp1 = dz[k + 1];
+  double indxp = 1.0 / (dx[i] + dx[i + 1]), indyp = 1.0 / (dy[j] + dy[j + 1]);
+  double indzp = 1.0 / (dz[k] + dz[k + 1]);
+#pragma unroll 1
+  for (int ax = 0; ax < 3; ++ax) {
+    if (D == 2 && ax == 1) continue;
+    // this axis: neighbour across the + face, the - face raw neighbour, spacing
+    const long qn = ax == 0 ? q.e : ax == 1 ? q.n : q.t;
+    const long qm = ax == 0 ? q.imjk : ax == 1 ? q.ijmk : q.ijkm;
+    const double d0 = ax == 0 ? dxi : ax == 1 ? dyj : dzk;
+    const double d1 = ax == 0 ? dxip1 : ax == 1 ? dyjp1 : dzkp1;
+    const double ind = ax == 0 ? indxp : ax == 1 ? indyp : indzp;
+    const double *vgx = ax == 0 ? ug : ax == 1 ? vg : wg;
+    const double *vsx = ax == 0 ? us : ax == 1 ? vs : ws;
+    // momentum fluxes of this component (stored in FT), differenced in double
+    const FT *m0 = (const FT *)F.mf[ax][0], *m1 = (const FT *)F.mf[ax][1], *m2 = (const FT *)F.mf[ax][2];
+    double epn_f = 1.0;
+    if (K.pmodel == 1) epn_f = (d0 * epn[qn] + d1 * epn[c]) * ind;
+    double t;
+    double r0, rs[NS];
+    {
+      double fx = (double)m0[c] - (double)m0[q.imjk];
+      double fz = (double)m2[c] - (double)m2[q.ijkm];
+      double fy = (D == 3) ? (double)m1[c] - (double)m1[q.ijmk] : 0.0;
+      t = 0.0;
+      if (ax == 0) {
+        t = t - indxp * 2.0 * fx * g.inrb[i];
+        t = t + (1.0 - ag) * (dx[i + 1] * rgpn[c] + dx[i] * rgpn[q.e]) * indxp * K.gravx;
+        t = t - g.indy[j] * fy;
+        t = t - g.indz[k] * fz;
+        t = t + (1.0 - alpha) * indxp * 2.0 * epn_f * (pn[c] - pn[q.e]);
+        r0 = F.f[PDAC_RUGN][c] + dt * t;
+      } else if (ax == 1) {
+        t = t + (1.0 - ag) * (dy[j + 1] * rgpn[c] + dy[j] * rgpn[q.n]) * indyp * K.gravy;
+        t = t - g.indx[i] * fx;
+        t = t - indyp * 2.0 * fy;
+        t = t - g.indz[k] * fz;
+        t = t + (1.0 - alpha) * indyp * 2.0 * epn_f * (pn[c] - pn[q.n]);
+        r0 = F.f[PDAC_RVGN][c] + dt * t;
+      } else {
+        t = t + (1.0 - ag) * (dz[k + 1] * rgpn[c] + dz[k] * rgpn[q.t]) * indzp * K.gravz;
+        t = t - g.indx[i] * fx * g.inr[i];
+        t = t - g.indy[j] * fy;
+        t = t - indzp * 2.0 * fz;
+        t = t + (1.0 - alpha) * indzp * 2.0 * epn_f * (pn[c] - pn[q.t]);
+        r0 = F.f[PDAC_RWGN][c] + dt * t;
+      }
+    }
+#pragma unroll
+    for (int is = 0; is < NS; ++is) {
+      const long o = (long)(is + 1) * n, os = (long)is * n;
+      double fx = (double)m0[o + c] - (double)m0[o + q.imjk];
+      double fz = (double)m2[o + c] - (double)m2[o + q.ijkm];
+      double fy = (D == 3) ? (double)m1[o + c] - (double)m1[o + q.ijmk] : 0.0;
+      double epsn_f = 0.0;
+      if (K.pmodel == 1) epsn_f = (d0 * rlkn[os + qn] + d1 * rlkn[os + c]) * ind * K.inrl[is];
+      const double sink = 0.0;
+      t = 0.0;
+      if (ax == 0) {
+        t = t - indxp * 2.0 * fx * g.inrb[i];
+        t = t + (1.0 - ag) * indxp * K.gravx * (dx[i + 1] * rlkn[os + c] + dx[i] * rlkn[os + q.e]);
+        t = t - g.indy[j] * fy;
+        t = t - g.indz[k] * fz;
+        t = t + (1.0 - alpha) * indxp * 2.0 * epsn_f * (pn[c] - pn[q.e]);
+        rs[is] = F.f[PDAC_RUSN][os + c] + dt * t;
+      } else if (ax == 1) {
+        t = t + (1.0 - ag) * indyp * K.gravy * (dy[j + 1] * rlkn[os + c] + dy[j] * rlkn[os + q.n]);
+        t = t - g.indx[i] * fx;
+        t = t - indyp * 2.0 * fy;
+        t = t - g.indz[k] * fz;
+        t = t + (1.0 - alpha) * indyp * 2.0 * epsn_f * (pn[c] - pn[q.n]);
+        rs[is] = F.f[PDAC_RVSN][os + c] + dt * t;
+      } else {
+        t = t + (1.0 - ag) * indzp * K.gravz * (rlkn[os + c] * dz[k + 1] + rlkn[os + q.t] * dz[k]);
+        t = t - g.indx[i] * fx * g.inr[i];
+        t = t - g.indy[j] * fy;
+        t = t - indzp * 2.0 * fz;
+        t = t + (1.0 - alpha) * indzp * 2.0 * epsn_f * (pn[c] - pn[q.t]);
+        rs[is] = F.f[PDAC_RWSN][os + c] + dt * t;
+      }
+      rs[is] = rs[is] + dt * sink * vsx[os + c];
+    }
+    // relative particle-particle velocities along the axis (drag.f:216-218)
+    // and this axis's interphase matrix
+    double d_pp[NS][NS];
+#pragma unroll
+    for (int a = 0; a < NS; ++a)
+#pragma unroll
+      for (int b = 0; b < NS; ++b) {
+        long oa = (long)a * n, ob = (long)b * n;
+        d_pp[a][b] = fabs(vsx[oa + c] - vsx[ob + c] + vsx[oa + qm] - vsx[ob + qm]) * 0.5;
+      }
+    double am[NA];
+    inter_axis<NS>(kpgv, dt, con, d_pp, am);
+    double *app = F.f[PDAC_APPU + ax];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) app[(long)a * n + c] = am[a];
     // (1-alpha) interphase coupling, tilde.f:637-706.  The serial loop reads
     // the east/north/top entries before it reaches those cells: they hold the
     // previous call's values (zero after allocate_fluxes), unless the
     // neighbour is the cell itself.
-    const double *ou = F.appu_old[0], *ov = F.appu_old[1], *ow = F.appu_old[2];
-    auto AE = [&](int ls) { return q.e == c ? au[ls] : (old_zero ? 0.0 : ou[(long)ls * n + q.e]); };
-    auto AN = [&](int ls) { return q.n == c ? av[ls] : (old_zero ? 0.0 : ov[(long)ls * n + q.n]); };
-    auto AT = [&](int ls) { return q.t == c ? aw[ls] : (old_zero ? 0.0 : ow[(long)ls * n + q.t]); };
-    auto CNX = [&](int ls) { return (1.0 - alpha) * (dxi * AE(ls) + dxip1 * au[ls]) * indxp; };
-    auto CNY = [&](int ls) { return (1.0 - alpha) * (dyj * AN(ls) + dyjp1 * av[ls]) * indyp; };
-    auto CNZ = [&](int ls) { return (1.0 - alpha) * (dzk * AT(ls) + dzkp1 * aw[ls]) * indzp; };
-    rug = rug - CNX(0) * ug[c];
-    if (D == 3) rvg = rvg - CNY(0) * vg[c];
-    rwg = rwg - CNZ(0) * wg[c];
+    const double *oapp = F.appu_old[ax];
+    auto AN = [&](int ls) { return qn == c ? am[ls] : (old_zero ? 0.0 : oapp[(long)ls * n + qn]); };
+    auto CN = [&](int ls) { return (1.0 - alpha) * (d0 * AN(ls) + d1 * am[ls]) * ind; };
+    r0 = r0 - CN(0) * vgx[c];
 #pragma unroll
     for (int l = 2; l <= NP; ++l) {
       int ls1 = l * (l - 1) / 2;
@@ -784,38 +804,20 @@ __global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, do
       for (int ll = 1; ll <= l; ++ll) {
         int ls = ls1 + ll - 1;
         if (ll == 1) {
-          rug = rug - CNX(ls) * us[ol + c];
-          if (D == 3) rvg = rvg - CNY(ls) * vs[ol + c];
-          rwg = rwg - CNZ(ls) * ws[ol + c];
-          rus[l - 2] = rus[l - 2] - CNX(ls) * ug[c];
-          if (D == 3) rvs[l - 2] = rvs[l - 2] - CNY(ls) * vg[c];
-          rws[l - 2] = rws[l - 2] - CNZ(ls) * wg[c];
+          r0 = r0 - CN(ls) * vsx[ol + c];
+          rs[l - 2] = rs[l - 2] - CN(ls) * vgx[c];
         } else if (ll == l) {
-          rus[l - 2] = rus[l - 2] - CNX(ls) * us[ol + c];
-          if (D == 3) rvs[l - 2] = rvs[l - 2] - CNY(ls) * vs[ol + c];
-          rws[l - 2] = rws[l - 2] - CNZ(ls) * ws[ol + c];
+          rs[l - 2] = rs[l - 2] - CN(ls) * vsx[ol + c];
         } else {
           long oll = (long)(ll - 2) * n;
-          rus[l - 2] = rus[l - 2] - CNX(ls) * us[oll + c];
-          rus[ll - 2] = rus[ll - 2] - CNX(ls) * us[ol + c];
-          if (D == 3) {
-            rvs[l - 2] = rvs[l - 2] - CNY(ls) * vs[oll + c];
-            rvs[ll - 2] = rvs[ll - 2] - CNY(ls) * vs[ol + c];
-          }
-          rws[l - 2] = rws[l - 2] - CNZ(ls) * ws[oll + c];
-          rws[ll - 2] = rws[ll - 2] - CNZ(ls) * ws[ol + c];
+          rs[l - 2] = rs[l - 2] - CN(ls) * vsx[oll + c];
+          rs[ll - 2] = rs[ll - 2] - CN(ls) * vsx[ol + c];
         }
       }
     }
-  }
-  F.f[PDAC_RUG][c] = rug;
-  F.f[PDAC_RWG][c] = rwg;
-  if (D == 3) F.f[PDAC_RVG][c] = rvg;
+    F.f[PDAC_RUG + ax][c] = r0;
 #pragma unroll
-  for (int is = 0; is < NS; ++is) {
-    F.f[PDAC_RUS][(long)is * n + c] = rus[is];
-    F.f[PDAC_RWS][(long)is * n + c] = rws[is];
-    if (D == 3) F.f[PDAC_RVS][(long)is * n + c] = rvs[is];
+    for (int is = 0; is < NS; ++is) F.f[PDAC_RUS + ax][(long)is * n + c] = rs[is];
   }
 }
 
