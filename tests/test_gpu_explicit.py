@@ -14,6 +14,8 @@ correlation, the drag laws), which differ in the last bits.
   so states are re-synchronised only between steps);
 * whole steps against the reference's fixtures and against the checker;
 * the stage-start order (PDAC_ORDER_STAGE) against the checker in that order;
+* the chain of the reference order by its three kernels (fixed-point
+  iterates, serial anti-diagonals, generic);
 * a 48x96 eruption column over 20 steps against the checker;
 * the FP32-flux form against the FP64 form.
 """
@@ -121,6 +123,45 @@ def test_stage_order_matches_checker(name):
         assert rel(sg[k], sc[k]) <= TOL, (k, rel(sg[k], sc[k]))
     # and it is a different order from the reference's
     assert max(rel(sg[k].reshape(v.shape), v) for k, v in ref.items()) > 1e-10
+
+
+@pytest.mark.parametrize("mode", ["serial", "general"])
+@pytest.mark.parametrize("name", [n for n in EXPLICIT if "_3d" not in n])
+def test_chain_kernels_match_reference(name, mode, monkeypatch):
+    """The default evaluates the reference order's chain as a fixed point of
+    all-cells iterates; the one-lane-per-column serial chain (PDAC_CHAIN=
+    serial, the fallback of the fixed point) and the generic chain kernel
+    (PDAC_CHAIN=general) must give the fixtures' fields as well."""
+    print(f"[explicit chain {mode}] {name}", flush=True)
+    monkeypatch.setenv("PDAC_CHAIN", mode)
+    case, ref, _ = load(name)
+    g = gpu(case)
+    g.step(case.nsteps)
+    out = g.state()
+    g.close()
+    for k, v in ref.items():
+        assert rel(out[k].reshape(v.shape), v) <= TOL, (k, rel(out[k].reshape(v.shape), v))
+
+
+def test_chain_fixed_point_equals_serial(monkeypatch):
+    """Fixed-point iterates and the serial chain agree bit for bit on a 48x96
+    eruption column over 10 steps."""
+    from pdac_amd.cases import eruption_column
+    print("[explicit chain fixed point vs serial]", flush=True)
+    out = {}
+    for mode in ("serial", "fixed"):
+        if mode == "serial":
+            monkeypatch.setenv("PDAC_CHAIN", "serial")
+        else:
+            monkeypatch.delenv("PDAC_CHAIN", raising=False)
+        case = eruption_column(nx=48, nz=96, nsteps=10, dt=5.0e-5, vent_radius=10.0)
+        case.time_integration = 1
+        g = gpu(case)
+        g.step(case.nsteps)
+        out[mode] = g.state()
+        g.close()
+    for k in out["serial"]:
+        np.testing.assert_array_equal(out["fixed"][k], out["serial"][k], err_msg=k)
 
 
 def test_eruption_column_20_steps():
