@@ -653,7 +653,7 @@ __device__ __forceinline__ void inter_axis(const double *kpgv, double dt, const 
 // need every velocity component; the rest is done one axis at a time (the
 // axes share no accumulator), which keeps one axis's values live at once.
 template <int D, int NS, typename FT>
-__global__ void __launch_bounds__(kBlock) k_tilde(Grid g, Consts K, Fields F, double dt, int old_zero) {
+__global__ void __launch_bounds__(kBlock, D == 2 ? 3 : 2) k_tilde(Grid g, Consts K, Fields F, double dt, int old_zero) {
   long c = cell_id();
   if (c >= g.n) return;
   if (!computed(g.flag[c])) return;
