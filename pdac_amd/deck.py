@@ -189,7 +189,7 @@ class Case:
         tstop = self.time + self.nsteps * self.dt
         lines = []
         lines.append("&control")
-        lines.append(f" run_name = 'graft', job_type = '{self.job_type}',")
+        lines.append(f" run_name = 'pdac_amd', job_type = '{self.job_type}',")
         lines.append(" restart_mode = 'restart',")
         lines.append(f" time = {d(self.time)}, tstop = {d(tstop)}, dt = {d(self.dt)},")
         lines.append(f" lpr = 0, imr = 0, tpr = {d(1.0e6)}, tdump = {d(self.nsteps * self.dt)},")

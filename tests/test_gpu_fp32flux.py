@@ -4,8 +4,8 @@ The FP32-flux form evaluates the MUSCL reconstruction, the limiters and the
 face fluxes of tilde (momentum), ygas (species) and htilde (enthalpy) in
 single precision and stores those face fluxes in single precision; the
 pressure-velocity iteration (its mass fluxes included), the temperature
-inversion and every accumulation stay FP64.  The bar is the one the form is
-judged by: relative L2 error of p and of each particle volume fraction
+inversion and every accumulation stay FP64.  The bar of the form: relative L2
+error of p and of each particle volume fraction
 (rlk / rho_s) at most 1e-4 against the FP64 form after the same steps; the
 error of every state field is checked against a looser 1e-3 and reported.
 """
