@@ -217,10 +217,20 @@ __device__ __forceinline__ void mole(const Consts &k, const double *yg, double *
   for (int g = 0; g < k.ngas; ++g) xg[g] = yg[g] / k.gmw[k.gas_type[g] - 1] * molinv;
 }
 
-__device__ __forceinline__ double thermal_eosg(const Consts &k, double tg, double p, const double *xg) {
+// mixture molar mass of thermal_eosg
+__device__ __forceinline__ double mix_mass(const Consts &k, const double *xg) {
   double mg = 0.0;
   for (int g = 0; g < k.ngas; ++g) mg = mg + xg[g] * k.gmw[k.gas_type[g] - 1];
+  return mg;
+}
+
+// thermal_eosg with the mixture molar mass given
+__device__ __forceinline__ double thermal_eosg_m(double tg, double p, double mg) {
   return p / (8.31432 * tg) * mg;
+}
+
+__device__ __forceinline__ double thermal_eosg(const Consts &k, double tg, double p, const double *xg) {
+  return thermal_eosg_m(tg, p, mix_mass(k, xg));
 }
 
 // heat capacity of one species (J/(kg K)), hcapgs.f:33-85
