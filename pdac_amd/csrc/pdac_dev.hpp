@@ -48,6 +48,9 @@ struct Consts {
   double phi[kMaxS];
   double philim[kMaxS][kMaxS], epsl[kMaxS][kMaxS], epsu[kMaxS][kMaxS], dkf[kMaxS][kMaxS];
   double gmw[7], atm_ygc[7];
+  // pow(gmw_g / gmw_h, -0.25) for gas slots g, h (the mixing rule of
+  // viscon), evaluated once on the host: it depends on the species only
+  double visw[7][7];
   int gas_type[7];
   int ngas, muscl, lm, lv, inmax, maxout, pmodel, drag_model, icpc, ord;
 };
@@ -169,6 +172,11 @@ __host__ __device__ __forceinline__ long refl_diag2(const int32_t *fl, long base
 template <typename T>
 __device__ __forceinline__ T limiter(int t, T beta, T erre) {
   T l, m, bu;
+  // minmod, the usual choice, tested first
+  if (t == 2) {
+    m = erre < T(1) ? erre : T(1);
+    return m > T(0) ? m : T(0);
+  }
   switch (t) {
   case 1: l = T(2) * erre / (T(1) + erre); return l > T(0) ? l : T(0);
   case 2: m = erre < T(1) ? erre : T(1); return m > T(0) ? m : T(0);
@@ -330,7 +338,7 @@ __device__ __forceinline__ void viscon(const Consts &k, const double *xgc, doubl
       int th = k.gas_type[h] - 1;
       double aa = k.gmw[tg_] / k.gmw[th];
       double bb = mmug[g] / mmug[h];
-      double cc = 1.0 + sqrt(bb) * pow(aa, -0.25);
+      double cc = 1.0 + sqrt(bb) * k.visw[g][h];
       double phij = cc * cc / sqrt(8.0 * (1.0 + aa));
       sum = sum + xgc[h] * phij;
     }
