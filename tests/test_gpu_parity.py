@@ -31,16 +31,22 @@ CONVERGING = [c for c in ALL if c != "mixed_2d_limiters"]
 
 def gpu(case, mode="persistent"):
     """GPU solver; mode selects how the red-black sweeps are launched: all in
-    one persistent kernel, two launches per colour phase, or batches of such
-    launches replayed as HIP graphs."""
+    one persistent kernel, one launch per colour phase, or batches of such
+    launches replayed as HIP graphs.  "launches:N" also fixes the cells each
+    thread of the sweep kernel checks (N = 1..4), which the fixtures' small
+    meshes would otherwise leave at one."""
     import os
 
     from pdac_amd.gpu import gpu_solver
+    mode, _, cpt = mode.partition(":")
     os.environ["PDAC_ITER_MODE"] = mode
+    if cpt:
+        os.environ["PDAC_SWEEP_CPT"] = cpt
     try:
         return gpu_solver(case, device=0)
     finally:
         os.environ.pop("PDAC_ITER_MODE", None)
+        os.environ.pop("PDAC_SWEEP_CPT", None)
 
 
 def rel(a, b):
@@ -74,7 +80,7 @@ STAGES = [
 ]
 
 
-@pytest.mark.parametrize("mode", ["persistent", "launches", "graph"])
+@pytest.mark.parametrize("mode", ["persistent", "launches", "graph", "launches:2", "launches:3", "launches:4"])
 @pytest.mark.parametrize("name", ALL)
 def test_stagewise(name, mode):
     print(f"[stagewise] {name} {mode}", flush=True)
