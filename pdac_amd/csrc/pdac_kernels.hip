@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -94,6 +95,21 @@ __device__ __forceinline__ void wl_append(int32_t *cnt, int32_t *list, bool pred
 // grid barriers and an error flag
 enum Ctr { C_DONE = 0, C_NIT1 = 1, C_NIT = 2, C_PHB = 3, C_EOS = 4, C_NITF = 5, C_WLP = 8, C_BAR = 14,
            C_ERR = 15, C_N = 16 };
+
+// Whether the iteration of which the host has launched 'gate' sweeps has
+// ended, by the test the host applies after each batch of sweeps: a sweep
+// found its predecessor converged, the last launched sweep left no cell
+// unconverged, or maxout sweeps ran.  The step's kernels after the
+// iteration are launched behind each batch with this gate, before the host
+// has read the batch's counters: they do nothing unless the batch ended the
+// iteration.  gate < 0: no gate.
+__device__ __forceinline__ bool iter_ended(const int32_t *ctr, int gate, int maxout) {
+  if (gate < 0) return true;
+  if (ctr[C_DONE]) return true;
+  const int32_t *last = ctr + C_WLP + 2 * (gate % 3);
+  if (last[0] + last[1] == 0) return true;
+  return gate >= maxout;
+}
 
 }  // namespace
 
