@@ -34,19 +34,24 @@ def gpu(case, mode="persistent"):
     one persistent kernel, one launch per colour phase, or batches of such
     launches replayed as HIP graphs.  "launches:N" also fixes the cells each
     thread of the sweep kernel checks (N = 1..4), which the fixtures' small
-    meshes would otherwise leave at one."""
+    meshes would otherwise leave at one; "launches:N:late" reads the sweep
+    counters before the check's first reads, as on meshes too large for a
+    resident grid."""
     import os
 
     from pdac_amd.gpu import gpu_solver
-    mode, _, cpt = mode.partition(":")
+    mode, _, rest = mode.partition(":")
+    cpt, _, order = rest.partition(":")
     os.environ["PDAC_ITER_MODE"] = mode
     if cpt:
         os.environ["PDAC_SWEEP_CPT"] = cpt
+    if order:
+        os.environ["PDAC_SWEEP_EARLY"] = "0" if order == "late" else "1"
     try:
         return gpu_solver(case, device=0)
     finally:
-        os.environ.pop("PDAC_ITER_MODE", None)
-        os.environ.pop("PDAC_SWEEP_CPT", None)
+        for k in ("PDAC_ITER_MODE", "PDAC_SWEEP_CPT", "PDAC_SWEEP_EARLY"):
+            os.environ.pop(k, None)
 
 
 def rel(a, b):
@@ -80,7 +85,8 @@ STAGES = [
 ]
 
 
-@pytest.mark.parametrize("mode", ["persistent", "launches", "graph", "launches:2", "launches:3", "launches:4"])
+@pytest.mark.parametrize("mode", ["persistent", "launches", "graph", "launches:2", "launches:3", "launches:4",
+                                  "launches:3:late"])
 @pytest.mark.parametrize("name", ALL)
 def test_stagewise(name, mode):
     print(f"[stagewise] {name} {mode}", flush=True)
@@ -128,7 +134,7 @@ def test_momentum_fluxes(name):
             assert rel(got, want) <= 1e-14, (comp, face, rel(got, want))
 
 
-@pytest.mark.parametrize("mode", ["persistent", "launches", "graph"])
+@pytest.mark.parametrize("mode", ["persistent", "launches", "graph", "launches:2:late"])
 @pytest.mark.parametrize("name", ALL)
 def test_steps_match_checker(name, mode):
     print(f"[steps] {name} {mode}", flush=True)
