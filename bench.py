@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # rocprofv3 FETCH_SIZE and WRITE_SIZE over every dispatch of the default
 # workload (FP32-flux form), each scaled by what the counter reads for a copy
 # of known size (--copy-calibration), averaged over the launches that did work
-SWEEP_TRAFFIC_PER_LAUNCH = 16902705.0
+SWEEP_TRAFFIC_PER_LAUNCH = 16913454.0
 # order of enum pdac_kernel (include/pdacgpu.h)
 KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
            "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill",
