@@ -30,7 +30,9 @@
 extern "C" {
 #endif
 
-#define PDAC_MAX_SOLID 8
+/* particle classes the device kernels are built for (the reference allows
+ * up to max_nsolid = 16, dimensions.f:13) */
+#define PDAC_MAX_SOLID 4
 #define PDAC_MAX_GAS 7
 
 /* numerical status codes (>0) */

@@ -5,7 +5,7 @@ import ctypes as ct
 
 import numpy as np
 
-PDAC_MAX_SOLID = 8
+PDAC_MAX_SOLID = 4
 PDAC_MAX_GAS = 7
 
 PREC_FP64 = 0

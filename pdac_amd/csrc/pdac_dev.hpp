@@ -17,7 +17,7 @@
 
 namespace pdac {
 
-constexpr int kMaxS = 4;               // particle classes supported on device
+constexpr int kMaxS = PDAC_MAX_SOLID;  // particle classes supported on device
 constexpr int kMaxP = kMaxS + 1;
 constexpr int kMaxA = kMaxP * (kMaxP + 1) / 2;
 

@@ -1,10 +1,12 @@
 """The C-ABI library: it is built, it loads, it exports every entry point
 that include/pdacgpu.h declares, and it rejects bad arguments before it
-touches a device.  No kernel runs here."""
+touches a device.  No kernel runs here; the gpu-marked test creates a
+context, which needs a device."""
 import ctypes as ct
 import os
 import re
 
+import numpy as np
 import pytest
 
 from pdac_amd import abi
@@ -42,10 +44,11 @@ def test_library_exports_every_declared_symbol(lib):
 
 
 def test_structs_match_header_layout():
-    # pdac_dims: 8 int32; pdac_params: 12 doubles + 6*8 doubles + 7 doubles
+    # pdac_dims: 8 int32; pdac_params: 12 doubles + 6*4 doubles + 7 doubles
     # + 7 int32 + 12 int32
+    assert abi.PDAC_MAX_SOLID == 4
     assert ct.sizeof(abi.Dims) == 32
-    assert ct.sizeof(abi.Params) == 8 * (12 + 6 * 8 + 7) + 4 * (7 + 12) + 4
+    assert ct.sizeof(abi.Params) == 8 * (12 + 6 * 4 + 7) + 4 * (7 + 12) + 4
 
 
 @pytest.mark.parametrize("dims,msg", [
@@ -69,3 +72,24 @@ def test_ensemble_rejects_empty_member_list(lib):
     assert rc == abi.ST_BAD_ARGUMENT
     assert "ensemble" in lib.pdac_last_error().decode()
     assert not ens.value
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flag,msg", [(257, "immersed"), (512, "immersed"), (1024, "immersed"),
+                                      (3, "unknown"), (0, "unknown")])
+def test_set_grid_rejects_unsupported_flags(flag, msg):
+    """Cell kinds the kernels do not treat are refused before any kernel
+    runs: an immersed-boundary cell without forcing points would otherwise
+    be advanced as plain fluid (the reference stops, bdry.f:103-120)."""
+    from pdac_amd.cases import mixed_box
+    from pdac_amd.gpu import gpu_solver
+    from pdac_amd.solver import PdacError
+    case = mixed_box(nsteps=1, bottom=3)
+    flags = case.flags().copy()
+    interior = int(np.flatnonzero(flags == 1)[len(np.flatnonzero(flags == 1)) // 2])
+    flags[interior] = flag
+    case.flags = lambda: flags
+    with pytest.raises(PdacError) as e:
+        gpu_solver(case, device=0)
+    assert e.value.code == abi.ST_BAD_ARGUMENT
+    assert msg in str(e.value)

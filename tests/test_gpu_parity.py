@@ -156,11 +156,14 @@ TOL_REF = {"p": 1e-7, "rgp": 2e-7, "rog": 2e-7, "tg": 2e-7, "sieg": 2e-7, "rlk":
            "us": 3e-6, "vs": 3e-6, "ws": 3e-6}
 
 
+@pytest.mark.parametrize("mode", ["launches", "graph", "launches:2:late", "persistent"])
 @pytest.mark.parametrize("name", CONVERGING)
-def test_steps_match_reference(name):
-    print(f"[reference] {name}", flush=True)
+def test_steps_match_reference(name, mode):
+    """Every launch mode of the sweeps (the default, one launch per colour
+    phase, first) against the reference's fixtures."""
+    print(f"[reference] {name} {mode}", flush=True)
     case, ref, nits = load(name)
-    g = gpu(case)
+    g = gpu(case, mode)
     for _ in range(case.nsteps):
         g.step(1)
     out = g.state()
