@@ -500,6 +500,12 @@ static void hcapg(const pdac_params *prm, const double *gmw, double tg, double *
   if (prm->icpc == 0) { cpc[4] = 1810.0; cpc[5] = 1004.0; }
 }
 
+/* hcapg exported for the unit pins: cpc[7] per kilogram for temperature tg
+ * under prm->icpc and prm->tref, with the gas molar masses gmw[7] */
+void pdac_cpu_hcapg(const pdac_params *prm, const double *gmw, double tg, double *cpc) {
+  hcapg(prm, gmw, tg, cpc);
+}
+
 /* caloric_eosg, eosg.f:139-223; returns info (0 ok) */
 static int caloric_eosg(const pdac_params *prm, const double *gmw, int ng,
                         double sieg, const double *yg, double *tg, double *cg) {

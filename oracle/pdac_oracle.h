@@ -55,6 +55,8 @@ double pdac_cpu_sdrag(const pdac_params *prm, int is, double du, double dv,
 int pdac_cpu_caloric_eosg(const pdac_params *prm, int ngas, double sieg,
                           const double *ygc, double *tg, double *cg);
 void pdac_cpu_gauss_jordan(int np, const double *plim, double *a, double *b);
+double pdac_cpu_extrapolate(double f2, double f1, double d12, double dx, int order);
+void pdac_cpu_hcapg(const pdac_params *prm, const double *gmw, double tg, double *cpc);
 int pdac_cpu_get_momflux(pdac_cpu_ctx *ctx, int comp, int face, double *out);
 
 #ifdef __cplusplus

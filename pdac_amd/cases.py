@@ -124,18 +124,20 @@ def _stretched(n, d0, d1, n0=1):
 
 def eruption_column(nx=512, nz=1024, nsolid=2, seed=1234, dr0=1.0, dr1=20.0,
                     vent_radius=50.0, w_vent=150.0, t_vent=1100.0, eps_vent=0.01,
-                    nsteps=5000, dt=None, top=2, east=4, **kw) -> Case:
+                    nsteps=5000, dt=None, top=2, east=4, vent="pulse", **kw) -> Case:
     """2D cylindrical collapsing eruption column.
 
     Radial cells stretch from ``dr0`` at the axis to ``dr1``; vertical cells
     likewise.  Stratified standard atmosphere (gravity -9.81), air + water
-    vapour, particle classes 50 and 500 um (2500 kg/m3).  The vent occupies
-    the lowest fluid row out to ``vent_radius``: gas w = ``w_vent`` with
-    seeded 2 % noise, T = ``t_vent``, particle fraction ``eps_vent`` split
-    evenly, water vapour only.  The axis is a slip wall, the ground no-slip,
-    the outer boundary free in/outflow; the top is a slip lid by default
-    (the reference's free in/outflow top reads a stale particle index,
-    bdry.f:561)."""
+    vapour, particle classes 50 and 500 um (2500 kg/m3).  The vent reaches
+    out to ``vent_radius``: gas w = ``w_vent`` with seeded 2 % noise per
+    cell, T = ``t_vent``, particle fraction ``eps_vent`` split evenly, water
+    vapour only.  ``vent="inlet"``: a sustained eruption, the vent cells are
+    a FIXED_FLOWS block of inlet cells (type 5, flag 16) in the ground row
+    whose values stay fixed; ``vent="pulse"``: the same values set once in
+    the lowest fluid row (an initial impulse).  The axis is a slip wall, the
+    ground no-slip, the outer boundary free in/outflow; the top is a slip
+    lid by default."""
     dx = _stretched(nx, dr0, dr1)
     dz = _stretched(nz, dr0, dr1)
     if dt is None:
@@ -161,7 +163,12 @@ def eruption_column(nx=512, nz=1024, nsolid=2, seed=1234, dr0=1.0, dr1=20.0,
     ygc = np.stack([np.zeros(n), np.ones(n)])
     wg = np.zeros(n)
     ws = np.zeros((nsolid, n))
-    vent = (kk == 1) & (ii >= 1) & (r[ii] <= vent_radius)
+    row = 0 if vent == "inlet" else 1
+    vent_cells = (kk == row) & (ii >= 1) & (ii <= nx - 2) & (r[ii] <= vent_radius)
+    if vent == "inlet":
+        iv = int(np.max(ii[vent_cells])) + 1
+        case.blocks = [{"typ": 5, "x": [2, iv], "z": [1, 1]}]
+    vent = vent_cells
     noise = rng.uniform(-1.0, 1.0, size=n)
     wg[vent] = w_vent * (1.0 + 0.02 * noise[vent])
     for s_ in range(nsolid):
@@ -227,9 +234,10 @@ def summit_vent_3d(nx=256, ny=256, nz=192, d=10.0, nsolid=3, seed=1234, vent_hal
 
 
 def mixed_box(three_d=False, nx=10, ny=7, nz=9, nsolid=2, seed=7, itc=0,
-              gravz=-9.81, nsteps=2, dt=2.0e-4, **kw) -> Case:
+              gravz=-9.81, nsteps=2, dt=2.0e-4, clean_top=0, **kw) -> Case:
     """Small randomised domain for parity tests: non-uniform cells, several
-    particle classes, seeded smooth random fields, walls of both kinds."""
+    particle classes, seeded smooth random fields, walls of both kinds.
+    ``clean_top`` rows below the lid start without particles."""
     rng = np.random.default_rng(seed)
     dx = rng.uniform(0.8, 1.25, nx)
     dz = rng.uniform(0.8, 1.25, nz)
@@ -253,6 +261,9 @@ def mixed_box(three_d=False, nx=10, ny=7, nz=9, nsolid=2, seed=7, itc=0,
     p = smooth(0.9e5, 1.3e5)
     tg = smooth(300.0, 900.0)
     eps = np.stack([smooth(0.0, 0.02) for _ in range(nsolid)])
+    if clean_top:
+        kk = np.repeat(np.arange(nz), (ny if three_d else 1) * nx)
+        eps[:, kk >= nz - 1 - clean_top] = 0.0
     ts = np.stack([tg + smooth(-50.0, 50.0) for _ in range(nsolid)])
     y0 = smooth(0.0, 1.0)
     ygc = np.stack([y0, 1.0 - y0])

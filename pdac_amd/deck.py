@@ -83,6 +83,12 @@ class Case:
     nsteps: int = 1
     time: float = 0.0
     cstop: bool = False
+    # FIXED_FLOWS blocks (input.f:832-906): dicts with 'typ' (cell flag
+    # 2**(typ-1), grid.f:415-434) and 1-based inclusive bounds 'x', 'y', 'z'
+    # as (lo, hi); typ 1, 5 and 8 also carry the block's fixed values, which
+    # a restart run reads but does not apply (setup.f:106-140: the restart
+    # file holds every cell's values)
+    blocks: List[dict] = field(default_factory=list)
     # initial primary fields, keyed like RestartState.fields
     fields: Dict[str, np.ndarray] = field(default_factory=dict)
 
@@ -127,6 +133,13 @@ class Case:
             fl[:, :, nx - 1] = 2 ** (self.east - 1)
             fl[:, 0, :] = 2 ** (self.south - 1)
             fl[:, ny - 1, :] = 2 ** (self.north - 1)
+        for b in self.blocks:
+            (xl, xh), (zl, zh) = b["x"], b["z"]
+            yl, yh = b["y"] if self.three_d else (1, 1)
+            v = 2 ** (b["typ"] - 1)
+            if self.three_d and b["typ"] == 8:
+                v = C.DOME_CELL
+            fl[zl - 1:zh, yl - 1:yh, xl - 1:xh] = v
         q = self.bl_row0
         for j in range(fl.shape[1]):
             for i in range(nx):
@@ -137,6 +150,35 @@ class Case:
                     elif col[k] == C.FLUID and (k + 1) <= q + 3:
                         col[k] = C.BL_CELL
         return fl.reshape(-1)
+
+    def _block_values(self, b) -> List[str]:
+        """The three value records of a fixed-flow block (input.f:845-868),
+        taken from the initial fields at the block's first cell."""
+        def d(x):
+            return repr(float(x)).replace("e", "D") if "e" in repr(float(x)) else repr(float(x)) + "D0"
+        f = self.fields
+        yl = b["y"][0] if self.three_d else 1
+        c = (b["x"][0] - 1) + (yl - 1) * self.nx + (b["z"][0] - 1) * self.nx * self.ny
+        g = lambda k, s=None: (0.0 if k not in f else  # noqa: E731
+                               float(np.asarray(f[k]).reshape(-1, self.ntot)[s or 0][c]))
+        rl = np.asarray(self.density, dtype=np.float64)
+        eps = [g("rlk", s) / rl[s] for s in range(self.nsolid)]
+        gas = 1.0 - sum(eps)
+        rows = []
+        if self.three_d:
+            rows.append(" " + " ".join(d(x) for x in (g("ug"), g("vg"), g("wg"), g("p"), gas, g("tg"))))
+            rows.append(" " + " ".join(" ".join(d(x) for x in (g("us", s), g("vs", s), g("ws", s), eps[s],
+                                                                   g("ts", s))) for s in range(self.nsolid)))
+        else:
+            rows.append(" " + " ".join(d(x) for x in (g("ug"), g("wg"), g("p"), gas, g("tg"))))
+            rows.append(" " + " ".join(" ".join(d(x) for x in (g("us", s), g("ws", s), eps[s], g("ts", s)))
+                                       for s in range(self.nsolid)))
+        conc = [0.0] * 7
+        ygc = np.asarray(f.get("ygc", np.zeros((len(self.gas_type), self.ntot)))).reshape(-1, self.ntot)
+        for k, t in enumerate(self.gas_type):
+            conc[t - 1] = float(ygc[k][c])
+        rows.append(" " + " ".join(d(x) for x in conc))
+        return rows
 
     # ------------------------------------------------------------------
     def scalars(self) -> dict:
@@ -258,6 +300,15 @@ class Case:
             lines += rows(dy)
         lines += rows(dz)
         lines.append("'FIXED_FLOWS'")
-        lines.append(" 0")
+        lines.append(f" {len(self.blocks)}")
+        for b in self.blocks:
+            (xl, xh), (zl, zh) = b["x"], b["z"]
+            if self.three_d:
+                (yl, yh) = b["y"]
+                lines.append(f" {b['typ']} {xl} {xh} {yl} {yh} {zl} {zh}")
+            else:
+                lines.append(f" {b['typ']} {xl} {xh} {zl} {zh}")
+            if b["typ"] in (1, 5, 8):
+                lines.extend(self._block_values(b))
         with open(path, "w") as fh:
             fh.write("\n".join(lines) + "\n")

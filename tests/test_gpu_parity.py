@@ -21,7 +21,7 @@ import pytest
 
 from oracle.binding import cpu_solver
 from pdac_amd import abi
-from tests.golden.make_golden import CASES, load
+from tests.golden.make_golden import CASES, STALE_TOP_GHOST, load, top_ghost_cells
 
 pytestmark = pytest.mark.gpu
 
@@ -167,5 +167,12 @@ def test_steps_match_reference(name, mode):
     for _ in range(case.nsteps):
         g.step(1)
     out = g.state()
+    ghost = top_ghost_cells(case)
     for k, v in ref.items():
-        assert rel(out[k].reshape(v.shape), v) <= TOL_REF[k], (k, rel(out[k].reshape(v.shape), v))
+        o = out[k].reshape(v.shape)
+        if name in STALE_TOP_GHOST and k == "ws":
+            # the reference's read past the end of ws (bdry.f:561)
+            o, v = o.copy(), v.copy()
+            o.reshape(-1, case.ntot)[:, ghost] = 0.0
+            v.reshape(-1, case.ntot)[:, ghost] = 0.0
+        assert rel(o, v) <= TOL_REF[k], (k, rel(o, v))

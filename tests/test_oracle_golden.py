@@ -6,11 +6,16 @@ import pytest
 
 from oracle.binding import cpu_solver
 from pdac_amd import abi
-from tests.golden.make_golden import CASES, load
+from tests.golden.make_golden import CASES, STALE_TOP_GHOST, load, top_ghost_cells
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_oracle_matches_reference(name):
+    """Bit for bit, except where the reference reads past the end of ws
+    for the ghost row above a free or zero-gradient lid (bdry.f:561): those
+    ghost values are left out, and the rest, which that read reaches only
+    through the top row's particle momentum flux, agree to 1e-12 relative
+    (bit for bit where no particle reaches the lid rows)."""
     case, ref, nits = load(name)
     s = cpu_solver(case, iter_order=abi.ORDER_LEXICOGRAPHIC)
     got_nits = []
@@ -19,8 +24,37 @@ def test_oracle_matches_reference(name):
         got_nits.append(s.nit_last)
     assert got_nits == nits
     out = s.state()
+    stale = name in STALE_TOP_GHOST
+    ghost = top_ghost_cells(case)
     for k, v in ref.items():
-        np.testing.assert_array_equal(out[k].reshape(v.shape), v, err_msg=k)
+        o = out[k].reshape(v.shape)
+        if not stale:
+            np.testing.assert_array_equal(o, v, err_msg=k)
+            continue
+        o, v = o.copy(), v.copy()
+        if k == "ws":
+            o.reshape(-1, case.ntot)[:, ghost] = 0.0
+            v.reshape(-1, case.ntot)[:, ghost] = 0.0
+        scale = max(np.max(np.abs(v)), 1e-300)
+        assert np.max(np.abs(o - v)) / scale <= 1e-12, (k, np.max(np.abs(o - v)) / scale)
+
+
+def test_stale_lid_ghost_is_the_only_mismatch_without_particles_at_the_lid():
+    """Where no particle reaches the lid rows the out-of-bounds read
+    cannot reach a computed cell: every other value is bit for bit."""
+    for name in ("mixed_2d_freetop_ord1", "mixed_2d_zerograd"):
+        case, ref, _ = load(name)
+        s = cpu_solver(case, iter_order=abi.ORDER_LEXICOGRAPHIC)
+        s.step(case.nsteps)
+        out = s.state()
+        ghost = top_ghost_cells(case)
+        for k, v in ref.items():
+            o = out[k].reshape(v.shape).copy()
+            v = v.copy()
+            if k == "ws":
+                o.reshape(-1, case.ntot)[:, ghost] = 0.0
+                v.reshape(-1, case.ntot)[:, ghost] = 0.0
+            np.testing.assert_array_equal(o, v, err_msg=f"{name} {k}")
 
 
 def test_golden_fixture_is_pinned_not_trivial():

@@ -134,3 +134,84 @@ def test_restart_round_trip():
     assert back.gas_type == st.gas_type and back.nx == st.nx
     for k, v in st.fields.items():
         np.testing.assert_array_equal(np.asarray(back.fields[k]).reshape(np.shape(v)), v)
+
+
+def test_ultrabeta_closed_form():
+    """Leonard's ULTRA-SHARP limiter with the beta scheme: psi = 2r up to
+    r = (1 - beta)/(2 - beta), the line (1 - beta) + beta r up to
+    r = (1 + beta)/beta, 2 beyond, 0 for r <= 0; psi(1) = 1."""
+    for beta in (0.25, 0.3, 0.5):
+        r1, r2 = (1 - beta) / (2 - beta), (1 + beta) / beta
+        for r in R:
+            psi = _lim(4, r, beta=beta)
+            if r <= 0:
+                want = 0.0
+            elif r <= r1:
+                want = 2 * r
+            elif r <= r2:
+                want = (1 - beta) + beta * r
+            else:
+                want = 2.0
+            assert psi == pytest.approx(want, rel=1e-15, abs=1e-15), (beta, r)
+        assert _lim(4, 1.0, beta=beta) == 1.0
+        assert 0.0 <= _lim(4, 5.0, beta=beta) <= 2.0
+
+
+def test_beta_unlimited_is_linear_in_r():
+    """The unlimited beta scheme: psi(0) = 1 - beta, psi(1) = 1, slope beta,
+    for every r (no TVD clipping)."""
+    for beta in (0.25, 0.3):
+        a, b = _lim(0, 0.0, beta=beta), _lim(0, 1.0, beta=beta)
+        assert a == pytest.approx(1 - beta, rel=1e-15) and b == 1.0
+        for r in (-7.0, -0.5, 3.0, 11.0):
+            assert _lim(0, r, beta=beta) == pytest.approx(a + (b - a) * r, rel=1e-14)
+
+
+def _extrapolate(f2, f1, d12, dx, order):
+    f = lib().pdac_cpu_extrapolate
+    f.restype = ct.c_double
+    f.argtypes = [ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_int]
+    return f(f2, f1, d12, dx, order)
+
+
+def test_extrapolate_order_zero_copies_and_order_one_is_exact_on_lines():
+    """The free in/outflow lid extrapolates the normal velocity: order 0
+    copies the last value, order 1 continues a straight line through the
+    last two points (exact on linear data, for any spacing)."""
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        a, b = rng.uniform(-50, 50, 2)
+        z1, d12, dx = rng.uniform(0, 100), rng.uniform(0.1, 20), rng.uniform(0.1, 20)
+        z2 = z1 + d12
+        f1, f2 = a + b * z1, a + b * z2
+        assert _extrapolate(f2, f1, d12, dx, 0) == f2
+        assert _extrapolate(f2, f1, d12, dx, 1) == pytest.approx(a + b * (z2 + dx), rel=1e-12, abs=1e-10)
+
+
+def _hcapg(case, tg):
+    f = lib().pdac_cpu_hcapg
+    f.argtypes = [ct.POINTER(abi.Params), ct.POINTER(ct.c_double), ct.c_double, ct.POINTER(ct.c_double)]
+    prm = abi.params_from_case(case)
+    gmw = np.ascontiguousarray(C.GMW, dtype=np.float64)
+    out = np.zeros(7)
+    f(ct.byref(prm), abi.dptr(gmw), tg, abi.dptr(out))
+    return out
+
+
+def test_heat_capacity_models():
+    """icpc 2: the polynomial fits grow with T for N2, H2O and air between
+    300 and 1500 K; icpc 1: every species at tref whatever T; icpc 0: water
+    vapour 1810 and air 1004 J/(kg K), constants."""
+    c2 = mixed_box(icpc=2)
+    lo, hi = _hcapg(c2, 300.0), _hcapg(c2, 1500.0)
+    assert hi[4] > lo[4] and hi[5] > lo[5]
+    c1 = mixed_box(icpc=1, tref=350.0)
+    for t in (250.0, 800.0, 2000.0):
+        np.testing.assert_array_equal(_hcapg(c1, t), _hcapg(c2, 350.0))
+    c0 = mixed_box(icpc=0, tref=298.0)
+    for t in (250.0, 800.0):
+        out = _hcapg(c0, t)
+        assert out[4] == 1810.0 and out[5] == 1004.0
+        np.testing.assert_array_equal(out[:4], _hcapg(c2, 298.0)[:4])
+    # the air polynomial at 298 K is close to the constant the model uses
+    assert _hcapg(c2, 298.0)[5] == pytest.approx(1004.0, rel=0.01)
