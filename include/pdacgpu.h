@@ -173,6 +173,24 @@ int pdac_step(pdac_ctx *ctx, double dt, int32_t rungekut, int32_t nsteps,
 int pdac_sync(pdac_ctx *ctx);
 const char *pdac_last_error(void);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): nsteps explicit steps as pdac_step takes them for
+ * time_integration = 1 (boundary, initialize_fieldn, per stage explicit_iter
+ * and ygas; same results), with the Runge-Kutta workspace in a static device
+ * array of PDAC_NOALLOC_BYTES that the library holds on each GPU it runs on
+ * until the process ends, instead of the context's own.  The call allocates
+ * nothing; the context needs pdac_set_grid and pdac_set_params only, with
+ * either time_integration (0 leaves it without a workspace of its own).
+ * Meshes whose workspace exceeds the array are refused with
+ * PDAC_ST_BAD_ARGUMENT (2D, two classes: about 200,000 cells).  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (from
+ * two host threads or on two streams) race.  eos_fail receives the largest
+ * caloric-EOS failure count of the steps, cfl the last Courant number. */
+#define PDAC_NOALLOC_BYTES 134217728
+int pdac_explicit_step_noalloc(pdac_ctx *ctx, double dt, int32_t rungekut, int32_t nsteps,
+                               int32_t *eos_fail, double *cfl);
+
 /* Ensembles of independent explicit-step domains of uneven size (no
  * counterpart in the reference, which runs one domain per job).  The members
  * are contexts set up as for pdac_step with time_integration = 1 and the

@@ -25,6 +25,7 @@ EXPORTS = [
     "pdac_profile_enable", "pdac_profile_select", "pdac_profile_reset", "pdac_profile_read",
     "pdac_profile_visits", "pdac_initialize_fieldn", "pdac_explicit_iter",
     "pdac_ens_create", "pdac_ens_destroy", "pdac_ens_tune", "pdac_ens_set_tiles", "pdac_ens_step",
+    "pdac_explicit_step_noalloc",
 ]
 
 
@@ -53,6 +54,9 @@ def lib() -> ct.CDLL:
         L.pdac_ens_set_tiles.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int32)]
         L.pdac_ens_step.argtypes = [ct.c_void_p, ct.POINTER(ct.c_double), ct.c_int32, ct.c_int32,
                                     ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double)]
+        L.pdac_explicit_step_noalloc.argtypes = [ct.c_void_p, ct.c_double, ct.c_int32, ct.c_int32,
+                                                 ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double)]
+        L.pdac_explicit_step_noalloc.restype = ct.c_int
         _lib = L
     return _lib
 
