@@ -72,6 +72,22 @@ struct pdac_cpu_ctx {
   double *snap_rgp, *snap_rlk;
   struct xstate *xs; /* explicit Runge-Kutta state, allocated on first use */
   int grid_set, params_set;
+  /* immersed boundaries (pdac_cpu_set_ib): forcing points of the x, y, z
+   * velocity components (TYPE forcing_point, immb.f:33-45), the forcing
+   * point of each cell (numx/y/z), the staggered-grid coordinates, the face
+   * masks and 1/vf (immb.f:47-60) and the filled_cell_2 toggles (fc2) */
+  int ib;
+  int nfp[3];
+  int32_t *fpi[3];     /* i, j, k, int, delta_i, delta_j, delta_k: 7 planes of nfp */
+  double *fpnsl[3];    /* no-slip point x, y, z: 3 planes of nfp */
+  int32_t *fpq[3], *fpqq[3];  /* 0-based interpolation neighbours (-1: none) */
+  double *fpvel[3];    /* pseudo-velocity of each phase: np planes of nfp */
+  int32_t *num[3];     /* per cell, 1-based forcing point or 0 */
+  double *crd[6];      /* x, xb, y, yb, z, zb, 0-based */
+  int32_t *bmask;      /* per cell: b_e, b_w, b_n, b_s, b_t, b_b (6 planes) */
+  double *ivf;
+  uint8_t *fc2;
+  int ib_off;          /* tests only: parts of the runtime left out (pdac_cpu_set_ib_off) */
 };
 
 static void x_free(struct xstate *x);
@@ -158,8 +174,96 @@ int pdac_cpu_destroy(pdac_cpu_ctx *s) {
   free(s->inrb); free(s->dy); free(s->indy); free(s->dz); free(s->indz);
   free(s->p_atm); free(s->t_atm); free(s->flag);
   x_free(s->xs);
+  for (int a = 0; a < 3; ++a) {
+    free(s->fpi[a]); free(s->fpnsl[a]); free(s->fpq[a]); free(s->fpqq[a]);
+    free(s->fpvel[a]); free(s->num[a]);
+  }
+  for (int a = 0; a < 6; ++a) free(s->crd[a]);
+  free(s->bmask); free(s->ivf); free(s->fc2);
   free(s);
   return 0;
+}
+
+/* Immersed boundaries: what set_forcing, local_forcing, fill_cells and
+ * face_fractions leave (immb.f:73-516, ghost.f:2888-3321).  fpi holds the
+ * integer fields of all points (x points, then y, then z) as 7 planes of
+ * nfp[0]+nfp[1]+nfp[2] (i, j, k, int, delta_i, delta_j, delta_k), fpd the
+ * no-slip points as 3 planes; coords are x, xb (nx each), y, yb (ny),
+ * z, zb (nz); bmask the face masks as 6 planes of ntot (e, w, n, s, t, b);
+ * ivf 1/vf per cell.  index_q/qq and numx/y/z follow local_forcing
+ * (ghost.f:2920-2940: a later point of a cell replaces an earlier one). */
+int pdac_cpu_set_ib(pdac_cpu_ctx *s, const int32_t *nfp, const int32_t *fpi, const double *fpd,
+                    const double *coords, const int32_t *bmask, const double *ivf) {
+  if (!s || !nfp || !s->three) return PDAC_ST_BAD_ARGUMENT;
+  long n = s->n;
+  int nx = s->nx, ny = s->ny, nz = s->nz;
+  long tot = (long)nfp[0] + nfp[1] + nfp[2];
+  long base = 0;
+  for (int a = 0; a < 3; ++a) {
+    int m = nfp[a];
+    s->nfp[a] = m;
+    s->fpi[a] = (int32_t *)malloc(sizeof(int32_t) * 7 * (size_t)(m + 1));
+    s->fpnsl[a] = (double *)malloc(sizeof(double) * 3 * (size_t)(m + 1));
+    s->fpq[a] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m + 1));
+    s->fpqq[a] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m + 1));
+    s->fpvel[a] = (double *)calloc((size_t)s->np * (m + 1), sizeof(double));
+    s->num[a] = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+    for (int f = 0; f < 7; ++f)
+      for (int p = 0; p < m; ++p) s->fpi[a][(long)f * m + p] = fpi[(long)f * tot + base + p];
+    for (int f = 0; f < 3; ++f)
+      for (int p = 0; p < m; ++p) s->fpnsl[a][(long)f * m + p] = fpd[(long)f * tot + base + p];
+    for (int p = 0; p < m; ++p) {
+      const int32_t *F = s->fpi[a];
+      int i = F[p], j = F[m + p], k = F[2 * m + p], it = F[3 * m + p];
+      int di = F[4 * m + p], dj = F[5 * m + p], dk = F[6 * m + p];
+      if (i < 1 || i > nx || j < 1 || j > ny || k < 1 || k > nz) return PDAC_ST_BAD_ARGUMENT;
+      long c = IJK(i, j, k);
+      s->num[a][c] = p + 1;
+      s->fpq[a][p] = (int32_t)IJK(i + di, j + dj, k + dk);
+      s->fpqq[a][p] = (it % 2 != 0 || it == 0) ? (int32_t)IJK(i + 2 * di, j + 2 * dj, k + 2 * dk) : -1;
+    }
+    base += m;
+  }
+  const int len[6] = {nx, nx, ny, ny, nz, nz};
+  long off = 0;
+  for (int a = 0; a < 6; ++a) {
+    s->crd[a] = (double *)malloc(sizeof(double) * (size_t)len[a]);
+    memcpy(s->crd[a], coords + off, sizeof(double) * (size_t)len[a]);
+    off += len[a];
+  }
+  s->bmask = (int32_t *)malloc(sizeof(int32_t) * 6 * (size_t)n);
+  memcpy(s->bmask, bmask, sizeof(int32_t) * 6 * (size_t)n);
+  s->ivf = (double *)malloc(sizeof(double) * (size_t)n);
+  memcpy(s->ivf, ivf, sizeof(double) * (size_t)n);
+  s->fc2 = (uint8_t *)calloc((size_t)n, 1);
+  s->ib = 1;
+  return 0;
+}
+
+/* Tests only: leave parts of the immersed-boundary runtime out, to show that
+ * the reference fixtures pin each (bit 0: the boundary forcing, 1: the
+ * tilde forcing, 2: the face masks and 1/vf, 3: the momentum stencil
+ * masks). */
+int pdac_cpu_set_ib_off(pdac_cpu_ctx *s, int bits) {
+  if (!s) return PDAC_ST_BAD_ARGUMENT;
+  s->ib_off = bits;
+  return 0;
+}
+
+/* faces, immb.f:1263-1277: the face masks and 1/vf of a cell (1 without
+ * immersed boundaries, face_init ghost.f:3322-3345) */
+static void cell_faces(const pdac_cpu_ctx *s, long c, double *be, double *bw, double *bn,
+                       double *bs, double *bt, double *bb, double *ivf) {
+  if (!s->ib || (s->ib_off & 4)) {
+    *be = *bw = *bn = *bs = *bt = *bb = 1.0;
+    *ivf = 1.0;
+    return;
+  }
+  long n = s->n;
+  *be = (double)s->bmask[c]; *bw = (double)s->bmask[n + c];
+  *bn = (double)s->bmask[2 * n + c]; *bs = (double)s->bmask[3 * n + c];
+  *bt = (double)s->bmask[4 * n + c]; *bb = (double)s->bmask[5 * n + c];
+  *ivf = s->ivf[c];
 }
 
 /* grid_setup, grid.f:284-340 */

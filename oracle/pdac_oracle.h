@@ -25,6 +25,9 @@ int pdac_cpu_set_grid(pdac_cpu_ctx *ctx, const double *dx, const double *dy,
                       const double *dz, const int32_t *flag,
                       const double *p_atm, const double *t_atm);
 int pdac_cpu_set_params(pdac_cpu_ctx *ctx, const pdac_params *prm);
+int pdac_cpu_set_ib(pdac_cpu_ctx *ctx, const int32_t *nfp, const int32_t *fpi, const double *fpd,
+                    const double *coords, const int32_t *bmask, const double *ivf);
+int pdac_cpu_set_ib_off(pdac_cpu_ctx *ctx, int bits);
 int pdac_cpu_put(pdac_cpu_ctx *ctx, int field, const double *host, int64_t n);
 int pdac_cpu_get(pdac_cpu_ctx *ctx, int field, double *host, int64_t n);
 int pdac_cpu_field_ncomp(const pdac_cpu_ctx *ctx, int field);

@@ -54,7 +54,11 @@ def atmosphere_profile(case):
         pbot, tbot = ptop, ttop
         ttop, ptop = hydrostatic(zbot, zt, grad, tbot, pbot, gravz)
         tops.append((zt, ptop, ttop, grad))
-    z = cell_heights(np.asarray(case.dz, dtype=np.float64))
+    if getattr(case, "itp", 0):
+        # the mesh lifted onto the topography (set_profile, topo.f:265-278)
+        z = case._topo()[0].mesh.z
+    else:
+        z = cell_heights(np.asarray(case.dz, dtype=np.float64))
     nz = len(z)
     p = np.empty(nz)
     t = np.empty(nz)

@@ -274,3 +274,88 @@ def mixed_box(three_d=False, nx=10, ny=7, nz=9, nsolid=2, seed=7, itc=0,
         kwv["vg"] = vel()
         kwv["vs"] = np.stack([vel() for _ in range(nsolid)])
     return fill_state(case, p, tg, ts, eps, ygc, **kwv)
+
+
+def summit_cone_3d(nx=256, ny=256, nz=192, d=10.0, nsolid=3, seed=1234, height=1500.0,
+                   sigma=1500.0, rough=30.0, vent_half=50.0, plateau=None, w_vent=100.0,
+                   t_vent=1000.0, eps_vent=0.02, nsteps=1000, dt=None, lpr=0, **kw) -> Case:
+    """3D Cartesian eruption from a square summit vent on a volcanic cone,
+    with immersed boundaries.
+
+    Uniform ``d`` cells; topography from a Gaussian cone of ``height`` and
+    width ``sigma`` plus seeded fractal roughness of amplitude ``rough``,
+    cut flat at the summit (a platform of half-width ``plateau``, default
+    twice the vent's) for the vent; the reference's topography set-up and
+    immersed boundaries (itp = 1, immb = 1).  The vent is a FIXED_FLOWS
+    block of inlet cells (type 5) in the platform's top solid layer, of
+    half-width ``vent_half``: gas w = ``w_vent`` with seeded 2 % noise per
+    cell, T = ``t_vent``, particle fraction ``eps_vent`` split evenly,
+    water vapour only.  Particle classes 30, 250 and 1000 um with densities
+    2700, 2000 and 1000 kg/m3; air + water vapour; stratified standard
+    atmosphere above the lifted mesh.  Sides free in/outflow, ground no-slip,
+    top a slip lid."""
+    if dt is None:
+        dt = 0.2 * d / (w_vent + 350.0)
+    plateau = 2.0 * vent_half if plateau is None else plateau
+    diam = [30.0, 250.0, 1000.0][:nsolid] + [100.0] * max(0, nsolid - 3)
+    dens = [2700.0, 2000.0, 1000.0][:nsolid] + [2500.0] * max(0, nsolid - 3)
+    case = Case(job_type="3D", itc=0, nx=nx, ny=ny, nz=nz, dx=np.full(nx, d), dy=np.full(ny, d),
+                dz=np.full(nz, d), west=4, east=4, south=4, north=4, bottom=3, top=2,
+                nsolid=nsolid, diameter=diam, density=dens, sphericity=[1.0] * nsolid,
+                viscosity=[0.5] * nsolid, specific_heat=[1200.0] * nsolid,
+                thermal_conductivity=[2.0] * nsolid, twophase_limit=[1.0e-8] * nsolid,
+                gravz=-9.81, stratification=True, dt=dt, nsteps=nsteps, itp=1, immb=1,
+                dem={"cell": d, "height": height, "sigma": sigma, "rough": rough,
+                     "plateau": plateau, "seed": seed},
+                dem_cellsize=d, lpr=lpr, **kw)
+    # the platform's surface in the upper quarter of a cell layer: the vent
+    # cells are that layer's cells (centre under the ground, top face at or
+    # above it), so the inflow crosses an open face into fluid cells
+    ts, _ = case._topo()
+    m = ts.mesh
+    vi = np.flatnonzero(np.abs(m.x) <= vent_half) + 1
+    vj = np.flatnonzero(np.abs(m.y) <= vent_half) + 1
+    if vi.size == 0 or vj.size == 0:
+        raise ValueError("no cell centre within the vent")
+    top = float(ts.topo2d[vi[0] - 1, vj[0] - 1])
+    kv = int(np.searchsorted(m.zb, top, side="left")) + 1  # first k with zb(k) >= top
+    # (the lifted mesh's heights are the DEM's: the lowest ground point is
+    # the bottom face)
+    case.dem = dict(case.dem, plateau_z=round(float(m.zb[kv - 1]) - 0.25 * d, 2))
+    case.__dict__.pop("_topo_cache", None)
+    ts, _ = case._topo()
+    m = ts.mesh
+    top = float(ts.topo2d[vi[0] - 1, vj[0] - 1])
+    kv = int(np.searchsorted(m.zb, top, side="left")) + 1
+    flat = ts.topo2d[vi - 1][:, vj - 1]
+    if flat.min() != flat.max() or not (m.z[kv - 1] < top <= m.zb[kv - 1]):
+        raise ValueError("the vent does not sit on a flat summit platform")
+    case.blocks = [{"typ": 5, "x": [int(vi[0]), int(vi[-1])], "y": [int(vj[0]), int(vj[-1])],
+                    "z": [kv, kv]}]
+    case.__dict__.pop("_topo_cache", None)
+    case.__dict__.pop("_ib_cache", None)
+    from .atmosphere import atmosphere_profile
+    p_atm, t_atm = atmosphere_profile(case)
+    rng = np.random.default_rng(seed)
+    n = case.ntot
+    kk = np.repeat(np.arange(nz), nx * ny)
+    jj = np.tile(np.repeat(np.arange(ny), nx), nz)
+    ii = np.tile(np.arange(nx), ny * nz)
+    p = p_atm[kk].copy()
+    tg = t_atm[kk].copy()
+    eps = np.zeros((nsolid, n))
+    ygc = np.stack([np.zeros(n), np.ones(n)])
+    wg = np.zeros(n)
+    ws = np.zeros((nsolid, n))
+    vent = ((kk == kv - 1) & (ii >= vi[0] - 1) & (ii <= vi[-1] - 1) &
+            (jj >= vj[0] - 1) & (jj <= vj[-1] - 1))
+    noise = rng.uniform(-1.0, 1.0, size=n)
+    wg[vent] = w_vent * (1.0 + 0.02 * noise[vent])
+    for s_ in range(nsolid):
+        ws[s_][vent] = wg[vent]
+        eps[s_][vent] = eps_vent / nsolid
+    tg[vent] = t_vent
+    ygc[0][vent] = 1.0
+    ygc[1][vent] = 0.0
+    ts_ = np.broadcast_to(tg, (nsolid, n)).copy()
+    return fill_state(case, p, tg, ts_, eps, ygc, wg=wg, ws=ws)

@@ -10,6 +10,7 @@ identical bits.
 """
 from __future__ import annotations
 
+import os
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
 
@@ -83,6 +84,15 @@ class Case:
     nsteps: int = 1
     time: float = 0.0
     cstop: bool = False
+    # topography (&topography itp, &boundaries immb): the DEM is a Gaussian
+    # cone made by pdac_amd.topography.cone_dem from the parameters in
+    # 'dem' (written as the reference's ASCII DEM file beside the deck)
+    itp: int = 0
+    immb: int = 0
+    dem: Optional[dict] = None
+    dem_cellsize: float = 10.0
+    dem_filtersize: float = 0.0
+    lpr: int = 0
     # FIXED_FLOWS blocks (input.f:832-906): dicts with 'typ' (cell flag
     # 2**(typ-1), grid.f:415-434) and 1-based inclusive bounds 'x', 'y', 'z'
     # as (lo, hi); typ 1, 5 and 8 also carry the block's fixed values, which
@@ -108,8 +118,54 @@ class Case:
         return dx, dy, dz
 
     def flags(self) -> np.ndarray:
-        """Cell flags, i fastest: ``flic`` (grid.f:375-470) followed by the
-        near-ground layer of ``export_topography`` (topo.f:1116-1162).
+        """Cell flags, i fastest, as the reference's set-up leaves them:
+        ``flic`` (grid.f:375-470), with topography ``set_flags``
+        (topo.f:1031-1112), the near-ground layer of ``export_topography``
+        (topo.f:1116-1162) and, with immersed boundaries, the flags of
+        ``set_forcing`` and ``fill_cells`` (see :meth:`immersed`)."""
+        if self.itp:
+            if self.immb:
+                return self.immersed().flags.copy()
+            return self._topo()[1].copy()
+        from .topography import export_layer
+        return export_layer(self, self.flags_flic())
+
+    def dem_object(self):
+        from .topography import cone_dem
+        d = dict(self.dem or {})
+        dx, dy, _ = self.cell_sizes()
+        return cone_dem(0.5 * float(np.sum(dx)), 0.5 * float(np.sum(dy)), **d)
+
+    def _topo(self):
+        """(TopoSetup, flags after export_topography), computed once."""
+        cached = self.__dict__.get("_topo_cache")
+        if cached is None:
+            from .topography import export_layer, topo_setup
+            ts = topo_setup(self, self.dem_object())
+            cached = (ts, export_layer(self, ts.flags.copy()))
+            self.__dict__["_topo_cache"] = cached
+        return cached
+
+    def immersed(self):
+        """The immersed-boundary set-up (pdac_amd.immersed), computed once."""
+        cached = self.__dict__.get("_ib_cache")
+        if cached is None:
+            from .immersed import immersed_setup
+            ts, fl = self._topo()
+            cached = immersed_setup(self, ts, fl)
+            self.__dict__["_ib_cache"] = cached
+        return cached
+
+    def flags_flic(self) -> np.ndarray:
+        """Cell flags of ``flic`` (grid.f:375-470): boundary faces and
+        FIXED_FLOWS blocks.  The near-ground layer of export_topography
+        turns the fluid cells of each column that lie within three rows above
+        the last no-slip cell met so far into ``bl_cell`` (5), which disables
+        the MUSCL corrections there.  Its row counter is not initialised in
+        the reference and carries over from column to column; it starts at
+        ``bl_row0`` here (1, the bottom boundary row, which is what the
+        reference build does for a deck whose bottom is not a no-slip
+        wall).
 
         The latter turns the fluid cells of each column that lie within three
         rows above the last no-slip cell met so far into ``bl_cell`` (5),
@@ -140,15 +196,6 @@ class Case:
             if self.three_d and b["typ"] == 8:
                 v = C.DOME_CELL
             fl[zl - 1:zh, yl - 1:yh, xl - 1:xh] = v
-        q = self.bl_row0
-        for j in range(fl.shape[1]):
-            for i in range(nx):
-                col = fl[:, j, i]
-                for k in range(nz):
-                    if col[k] == C.NOSLIP_WALL:
-                        q = k + 1
-                    elif col[k] == C.FLUID and (k + 1) <= q + 3:
-                        col[k] = C.BL_CELL
         return fl.reshape(-1)
 
     def _block_values(self, b) -> List[str]:
@@ -185,7 +232,7 @@ class Case:
         """All non-field settings, JSON-serialisable."""
         out = {}
         for k, v in self.__dict__.items():
-            if k == "fields":
+            if k == "fields" or k.startswith("_"):
                 continue
             if isinstance(v, np.ndarray):
                 v = v.tolist()
@@ -234,7 +281,7 @@ class Case:
         lines.append(f" run_name = 'pdac_amd', job_type = '{self.job_type}',")
         lines.append(" restart_mode = 'restart',")
         lines.append(f" time = {d(self.time)}, tstop = {d(tstop)}, dt = {d(self.dt)},")
-        lines.append(f" lpr = 0, imr = 0, tpr = {d(1.0e6)}, tdump = {d(self.nsteps * self.dt)},")
+        lines.append(f" lpr = {self.lpr}, imr = 0, tpr = {d(1.0e6)}, tdump = {d(self.nsteps * self.dt)},")
         lines.append(f" nfil = 0, max_seconds = {d(1.0e7)}, cstop = {b(self.cstop)}")
         lines.append("/")
         lines.append("&model")
@@ -252,10 +299,16 @@ class Case:
         lines.append("&boundaries")
         lines.append(f" west = {self.west}, east = {self.east}, south = {self.south},")
         lines.append(f" north = {self.north}, bottom = {self.bottom}, top = {self.top},")
-        lines.append(f" immb = 0, ibl = 0, ord = {self.ord}")
+        lines.append(f" immb = {self.immb}, ibl = 0, ord = {self.ord}")
         lines.append("/")
         lines.append("&topography")
-        lines.append(" itp = 0")
+        if self.itp:
+            lines.append(f" itp = {self.itp}, dem_file = 'topo.dat', iavv = 0, ismt = 1,")
+            lines.append(f" cellsize = {d(self.dem_cellsize)}, filtersize = {d(self.dem_filtersize)},")
+            lines.append(" itrans = T, nocrater = T")
+            self.dem_object().write(os.path.join(os.path.dirname(os.path.abspath(path)), "topo.dat"))
+        else:
+            lines.append(" itp = 0")
         lines.append("/")
         lines.append("&inlet")
         lines.append(" ivent = 0")

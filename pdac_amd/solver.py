@@ -42,13 +42,42 @@ def bind(lib: ct.CDLL, prefix: str):
         "initialize_fieldn": [P, ct.c_double],
         "explicit_iter": [P, ct.c_double, ip, ct.POINTER(ct.c_double)],
     }
+    sig["set_ib"] = [P, ip, ip, dp, dp, ip, dp]
     out = {}
     for name, args in sig.items():
-        fn = getattr(lib, prefix + name)
+        fn = getattr(lib, prefix + name, None)
+        if fn is None and name == "set_ib":
+            continue  # a library without immersed boundaries
         fn.argtypes = args
         fn.restype = ct.c_int
         out[name] = fn
     return out
+
+
+def ib_arrays(case):
+    """The immersed-boundary arguments of set_ib for a case with
+    ``immb = 1`` (pdac_amd.immersed): point counts, the integer and
+    real planes of the forcing points, the grid coordinates, the face masks
+    and 1/vf."""
+    ib = case.immersed()
+    ts, _ = case._topo()
+    m = ts.mesh
+    nfp = np.array([len(f) for f in ib.fp], dtype=np.int32)
+    tot = int(nfp.sum())
+    fpi = np.empty((7, tot), dtype=np.int32)
+    fpd = np.empty((3, tot), dtype=np.float64)
+    o = 0
+    for f in ib.fp:
+        k = len(f)
+        for r, col in enumerate((f.i, f.j, f.k, f.int_, f.di, f.dj, f.dk)):
+            fpi[r, o:o + k] = col
+        for r, col in enumerate((f.nsx, f.nsy, f.nsz)):
+            fpd[r, o:o + k] = col
+        o += k
+    coords = np.concatenate([m.x, m.xb, m.y, m.yb, m.z, m.zb]).astype(np.float64)
+    bmask = np.ascontiguousarray(ib.bmask, dtype=np.int32)
+    ivf = np.ascontiguousarray(ib.ivf, dtype=np.float64)
+    return nfp, np.ascontiguousarray(fpi), np.ascontiguousarray(fpd), coords, bmask, ivf
 
 
 class Solver:
@@ -84,6 +113,14 @@ class Solver:
             self.ctx, abi.dptr(self._dx), abi.dptr(self._dy), abi.dptr(self._dz),
             abi.iptr(self._flag), abi.dptr(self._patm), abi.dptr(self._tatm)))
         self._check("set_params", self.fn["set_params"](self.ctx, ct.byref(self.params)))
+        if getattr(case, "immb", 0):
+            if "set_ib" not in self.fn:
+                raise PdacError(prefix + "set_ib", abi.ST_BAD_ARGUMENT, "library without immersed boundaries")
+            self._ib = ib_arrays(case)
+            nfp, fpi, fpd, coords, bmask, ivf = self._ib
+            self._check("set_ib", self.fn["set_ib"](
+                self.ctx, abi.iptr(nfp), abi.iptr(fpi), abi.dptr(fpd), abi.dptr(coords),
+                abi.iptr(bmask), abi.dptr(ivf)))
         self.time = case.time
         self.nit_last = 0
         self.nit_total = 0

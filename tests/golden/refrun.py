@@ -8,6 +8,8 @@ import shutil
 import subprocess
 import tempfile
 
+import numpy as np
+
 from pdac_amd.restart import read_restart
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -19,7 +21,22 @@ def available():
     return os.path.exists(PDACX)
 
 
-def run_reference(case, workdir=None, keep=False, timeout=3600):
+def read_forcing_points(wd):
+    """The forcing points the reference reports (set_forcing with lpr > 0,
+    immb.f:478-503): per component an array of (i, j, k, int) and one of
+    the no-slip points as printed (F12.3)."""
+    out = {}
+    for a in "xyz":
+        path = os.path.join(wd, f"fpt{a}.dat")
+        if not os.path.exists(path):
+            continue
+        rows = [l.split() for l in open(path) if l.strip()]
+        out[f"fpt{a}_int"] = np.array([[int(v) for v in r[1:5]] for r in rows], dtype=np.int32)
+        out[f"fpt{a}_nsl"] = np.array([[float(v) for v in r[5:8]] for r in rows])
+    return out
+
+
+def run_reference(case, workdir=None, keep=False, timeout=3600, extras=None):
     wd = workdir or tempfile.mkdtemp(prefix="pdacref_")
     os.makedirs(wd, exist_ok=True)
     case.write_deck(os.path.join(wd, "pdac.dat"))
@@ -30,6 +47,8 @@ def run_reference(case, workdir=None, keep=False, timeout=3600):
         raise RuntimeError(f"reference run failed in {wd}: {r.stdout[-2000:]} {r.stderr[-2000:]}")
     st = read_restart(os.path.join(wd, "pdac.res"), case.three_d)
     nits = [int(l.split("nit:")[1]) for l in log.splitlines() if "nit:" in l]
+    if extras is not None:
+        extras.update(read_forcing_points(wd))
     if not keep and workdir is None:
         shutil.rmtree(wd, ignore_errors=True)
     return st, nits

@@ -1,0 +1,88 @@
+"""The immersed-boundary set-up (pdac_amd.topography, pdac_amd.immersed)
+against what the reference's own set-up reported for the same decks: the
+forcing points it wrote (fptx/y/z.dat: cell, interpolation type, no-slip
+point), and, through the whole-step fixtures, the flags, masks and
+interpolation neighbours; plus the oracle's immersed-boundary runtime, each
+part of which the fixtures must pin."""
+import numpy as np
+import pytest
+
+from oracle.binding import cpu_solver
+from pdac_amd import abi, constants as C
+from pdac_amd.topography import Dem, fortran_nint, interp_2d
+from tests.golden.make_golden import load, load_extras
+
+IB_CASES = ["cone_3d_ib", "cone_3d_ib_steep"]
+
+
+@pytest.mark.parametrize("name", IB_CASES)
+def test_forcing_points_match_the_reference(name):
+    case, _, _ = load(name)
+    ref = load_extras(name)
+    ib = case.immersed()
+    for a, ax in enumerate("xyz"):
+        fp = ib.fp[a]
+        got = np.array([fp.i, fp.j, fp.k, fp.int_], dtype=np.int32).T
+        np.testing.assert_array_equal(got, ref[f"fpt{ax}_int"], err_msg=ax)
+        nsl = np.array([fp.nsx, fp.nsy, fp.nsz]).T
+        # printed with F12.3
+        assert np.max(np.abs(nsl - ref[f"fpt{ax}_nsl"])) <= 0.0005 + 1e-9, ax
+    # the set-up exercises every kind of point and cell
+    types = set(np.concatenate([np.array(f.int_) for f in ib.fp]).tolist())
+    assert 0 in types and any(t >= 20 for t in types) and any(1 <= t <= 8 for t in types)
+    fl = ib.flags
+    for f in (C.IMMB_CELL, C.FILLED_CELL_1, C.INLET_CELL):
+        assert np.any(fl == f), f
+
+
+@pytest.mark.parametrize("bit,what", [(1, "boundary forcing"), (2, "tilde forcing"),
+                                      (4, "face masks and 1/vf"), (8, "momentum stencil masks")])
+def test_fixtures_pin_each_part_of_the_runtime(bit, what):
+    """Leaving any part of the immersed-boundary runtime out of the oracle
+    must miss the reference fixtures."""
+    worst = 0.0
+    for name in IB_CASES:
+        case, ref, _ = load(name)
+        s = cpu_solver(case, iter_order=abi.ORDER_LEXICOGRAPHIC)
+        s.lib.pdac_cpu_set_ib_off(s.ctx, bit)
+        try:
+            s.step(case.nsteps)
+        except Exception:
+            worst = np.inf
+            continue
+        out = s.state()
+        worst = max(worst, max(np.max(np.abs(out[k].reshape(v.shape) - v)) / max(np.max(np.abs(v)), 1e-300)
+                               for k, v in ref.items()))
+    assert worst > 1e-6, what
+
+
+def test_dem_file_round_trip(tmp_path):
+    from pdac_amd.topography import cone_dem
+    d = cone_dem(100.0, 120.0, cell=10.0, height=50.0, sigma=40.0, rough=2.0, seed=3)
+    d.write(str(tmp_path / "topo.dat"))
+    back = Dem.read(str(tmp_path / "topo.dat"))
+    assert (back.xll, back.yll, back.cell) == (d.xll, d.yll, d.cell)
+    np.testing.assert_array_equal(back.z, d.z)
+    # written north to south (the reference reads rows j = ny..1)
+    first = open(tmp_path / "topo.dat").read().split("\n")[6].split()
+    assert float(first[0]) == d.z[-1, 0]
+
+
+def test_bilinear_interpolation_closed_forms():
+    """interp_2d reproduces the nodes, the average midway, and bilinear
+    functions exactly (up to rounding)."""
+    rng = np.random.default_rng(1)
+    x1 = np.arange(-50.0, 51.0, 10.0)
+    y1 = np.arange(-30.0, 41.0, 10.0)
+    a, b, c, d = rng.uniform(-3, 3, 4)
+    f1 = a + b * x1[:, None] + c * y1[None, :] + d * x1[:, None] * y1[None, :]
+    x2 = np.array([-45.0, -40.0, 3.3, 27.1])
+    y2 = np.array([-25.0, 0.0, 12.5, 33.0])
+    f2 = interp_2d(x1, y1, f1, x2, y2)
+    want = a + b * x2[:, None] + c * y2[None, :] + d * x2[:, None] * y2[None, :]
+    np.testing.assert_allclose(f2, want, rtol=1e-12, atol=1e-10)
+    assert interp_2d(x1, y1, f1, x1[1:-1], y1[1:-1]).tolist() == f1[1:-1, 1:-1].tolist()
+
+
+def test_nint_rounds_halves_away_from_zero():
+    assert fortran_nint([0.5, 1.5, 2.5, -0.5, -2.5, 2.4999]).tolist() == [1, 2, 3, -1, -3, 2]
