@@ -128,6 +128,24 @@ int pdac_set_grid(pdac_ctx *ctx, const double *dx, const double *dy,
                   const double *p_atm, const double *t_atm);
 int pdac_set_params(pdac_ctx *ctx, const pdac_params *prm);
 
+/* Immersed boundaries (immb = 1, 3D, semi-implicit): what the reference's
+ * set-up leaves in MODULE immersed_boundaries (immb.f:33-60): the forcing
+ * points of the x, y and z velocity components (TYPE forcing_point,
+ * immb.f:33-45; set_forcing :73-516), the face masks b_e_..b_b_ and ivf_
+ * (fill_cells ghost.f:3052-3261, face_fractions ghost.f:3263-3321).
+ * nfp[3] point counts; fpi 7 planes of nfp[0]+nfp[1]+nfp[2] int32 (i, j, k
+ * 1-based, int, delta_i, delta_j, delta_k; x points, then y, then z, each in
+ * the reference's order); fpd 3 planes of doubles (the no-slip point nsl
+ * x, y, z); coords x[nx], xb[nx], y[ny], yb[ny], z[nz], zb[nz] of the
+ * georeferenced, lifted mesh (compute_UTM_coords topo.f:731-751,
+ * set_profile :265-278); bmask 6 planes of ntot int32 (b_e, b_w, b_n, b_s,
+ * b_t, b_b); ivf[ntot].  The grid's flags (pdac_set_grid) are those after
+ * fill_cells; a grid with immb or filled cells needs this call before any
+ * step.  index_q/index_qq and numx/y/z follow local_forcing
+ * (ghost.f:2888-3050). */
+int pdac_set_ib(pdac_ctx *ctx, const int32_t *nfp, const int32_t *fpi, const double *fpd,
+                const double *coords, const int32_t *bmask, const double *ivf);
+
 /* Host <-> device copies of one field; n must equal ntot*ncomp(field). */
 int pdac_put(pdac_ctx *ctx, int field, const double *host, int64_t n);
 int pdac_get(pdac_ctx *ctx, int field, double *host, int64_t n);

@@ -68,6 +68,54 @@ struct Fields {
   int32_t *wl;                         // work list of one colour phase
   uint8_t *dirty;                      // cell must be checked again (iter)
   int32_t *stamp;                      // colour phase whose start snap_* holds
+  // immersed boundaries: per cell the face masks b_e..b_b and whether 1/vf
+  // differs from 1 (IBM_* bits), and 1/vf; null without immersed boundaries
+  const uint8_t *ibm;
+  const double *ivf;
+};
+
+// face masks of a cell (b_e_ .. b_b_, immb.f:56-58): bits of Fields::ibm
+enum : uint8_t { IBM_E = 1, IBM_W = 2, IBM_N = 4, IBM_S = 8, IBM_T = 16, IBM_B = 32, IBM_VF = 64 };
+constexpr uint8_t IBM_OPEN = 63;  // every face open, 1/vf = 1 (face_init, ghost.f:3322-3345)
+
+// faces, immb.f:1263-1277: the face masks and 1/vf of a cell, as doubles
+// (the reference multiplies its integer masks into double fluxes)
+struct FaceMask {
+  double e, w, n, s, t, b, ivf;
+};
+
+__device__ __forceinline__ FaceMask mask_of(uint8_t m, const double *ivf, long c) {
+  FaceMask r;
+  r.e = (m & IBM_E) ? 1.0 : 0.0;
+  r.w = (m & IBM_W) ? 1.0 : 0.0;
+  r.n = (m & IBM_N) ? 1.0 : 0.0;
+  r.s = (m & IBM_S) ? 1.0 : 0.0;
+  r.t = (m & IBM_T) ? 1.0 : 0.0;
+  r.b = (m & IBM_B) ? 1.0 : 0.0;
+  r.ivf = (m & IBM_VF) ? ivf[c] : 1.0;
+  return r;
+}
+
+__device__ __forceinline__ FaceMask cell_faces(const Fields &F, long c) {
+  return mask_of(F.ibm ? F.ibm[c] : IBM_OPEN, F.ivf, c);
+}
+
+// Forcing points of one velocity component on the device (TYPE
+// forcing_point, immb.f:33-45, with the geometry of velint3d,
+// interp.f:687-781, evaluated once on the host)
+struct IbAxis {
+  int nfp;
+  const int32_t *cell, *q, *qq;  // 0-based cell, first and second interpolation neighbour
+  const uint8_t *mode;           // 0: h/zA v(q) (int >= 20), 1: -h/zA v(q), 2: the smaller of two
+  const uint8_t *ext;            // int >= 20: homogeneous Neumann values (hn)
+  const double *hza, *zbh, *hza2, *zbza;  // h/zA, zB-h, h-zA, zB-zA
+  double *vel;                   // pseudo-velocity of each phase: NP planes of nfp
+};
+
+struct Ib {
+  IbAxis ax[3];
+  const int32_t *num[3];  // per cell, 1-based forcing point of each component, 0 none
+  uint8_t *fc2;           // filled_cell_2 toggles of hn (interp.f:876-916)
 };
 
 // --------------------------------------------------------------------

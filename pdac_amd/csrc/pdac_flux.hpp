@@ -157,6 +157,20 @@ __device__ __forceinline__ void gather_rnb(const FullIdx<D> &x, const double *a,
   }
 }
 
+// maskstencil(u, rnb(mask, b_, ijk)) (subscr.f:1548-1582, tilde.f:1072-1118):
+// each raw-neighbour entry of a velocity stencil times that cell's face mask
+// (bit 'bit' of ibm), on the entries gather_rnb fills
+template <int D, typename T>
+__device__ __forceinline__ void mask_rnb(const FullIdx<D> &x, const uint8_t *ibm, uint8_t bit, Stc<T> &o) {
+  auto mk = [&](long c) { return (ibm[c] & bit) ? T(1) : T(0); };
+  o.c = o.c * mk(x.c); o.e = o.e * mk(x.e); o.w = o.w * mk(x.w); o.t = o.t * mk(x.t); o.b = o.b * mk(x.b);
+  o.ee = o.ee * mk(x.ee); o.tt = o.tt * mk(x.tt); o.wt = o.wt * mk(x.wt); o.eb = o.eb * mk(x.eb);
+  if (D == 3) {
+    o.n = o.n * mk(x.n); o.s = o.s * mk(x.s); o.nn = o.nn * mk(x.nn); o.es = o.es * mk(x.es);
+    o.st = o.st * mk(x.st); o.wn = o.wn * mk(x.wn); o.nb = o.nb * mk(x.nb);
+  }
+}
+
 // ----------------------------------------------------------------------
 // staggered densities, interp.f:14-320 (entries the momentum fluxes read)
 template <int D, typename T>

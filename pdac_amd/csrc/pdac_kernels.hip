@@ -411,6 +411,87 @@ __global__ void __launch_bounds__(kBlock) k_boundary(Grid g, Consts K, Fields F)
   if (c < g.n) boundary_cell<D, NS>(g, K, F, c);
 }
 
+// hn, interp.f:832-918: homogeneous Neumann values of a filled cell from
+// its forcing neighbour (m the forcing cell, nn its neighbour along the
+// component); a filled_cell_2 sums halves of its two forcing points
+template <int NS>
+__device__ __forceinline__ void hn_d(const Grid &g, const Consts &K, Fields &F, const Ib &ib, long m, long nn) {
+  const long n = g.n;
+  const int32_t fm = g.flag[m], fn = g.flag[nn];
+  auto each = [&](auto &&op) {
+    for (int id : {PDAC_P, PDAC_EP, PDAC_RGP, PDAC_TG, PDAC_SIEG}) op(F.f[id]);
+#pragma unroll
+    for (int is = 0; is < NS; ++is) {
+      op(F.f[PDAC_RLK] + (long)is * n);
+      op(F.f[PDAC_TS] + (long)is * n);
+      op(F.f[PDAC_SIES] + (long)is * n);
+    }
+    for (int g_ = 0; g_ < K.ngas; ++g_) op(F.f[PDAC_YGC] + (long)g_ * n);
+  };
+  if (fm == FILLED_CELL_1) each([&](double *a) { a[m] = a[nn]; });
+  else if (fn == FILLED_CELL_1) each([&](double *a) { a[nn] = a[m]; });
+  if (fm == FILLED_CELL_2) {
+    if (!ib.fc2[m]) each([&](double *a) { a[m] = 0.5 * a[nn]; });
+    else each([&](double *a) { a[m] = a[m] + 0.5 * a[nn]; });
+    ib.fc2[m] = ib.fc2[m] ? 0 : 1;
+  } else if (fn == FILLED_CELL_2) {
+    if (!ib.fc2[nn]) each([&](double *a) { a[nn] = 0.5 * a[m]; });
+    else each([&](double *a) { a[nn] = a[nn] + 0.5 * a[m]; });
+    ib.fc2[nn] = ib.fc2[nn] ? 0 : 1;
+  }
+}
+
+// the immersed-boundary part of boundary (bdry.f:103-203): pseudo-velocities
+// of the forcing points of an immb or filled cell (velint3d, interp.f:687-781)
+template <int D, int NS>
+__device__ __forceinline__ void ib_boundary_cell(const Grid &g, const Consts &K, Fields &F, const Ib &ib, long c) {
+  const long n = g.n;
+  const long nxt[3] = {c + 1, c + (long)g.nx, c + (long)g.nx * g.ny};  // ipjk, ijpk, ijkp
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (D == 2 && a == 1) continue;
+    const int pt = ib.num[a][c] - 1;
+    if (pt < 0) continue;
+    const IbAxis &A = ib.ax[a];
+    double *vg = F.f[PDAC_UG + a], *vs = F.f[PDAC_US + a];
+    const long q = A.q[pt], qq = A.qq[pt];
+    const int mode = A.mode[pt];
+    const double hza = A.hza[pt];
+    auto one = [&](const double *v) {
+      if (mode == 0) return hza * v[q];
+      if (mode == 1) return -hza * v[q];
+      const double v2 = -(A.zbh[pt] * v[q] + A.hza2[pt] * v[qq]) / A.zbza[pt];
+      const double v1 = -hza * v[q];
+      return fabs(v1) < fabs(v2) ? v1 : v2;
+    };
+    double vel[NS + 1];
+    vel[0] = one(vg);
+#pragma unroll
+    for (int is = 0; is < NS; ++is) vel[1 + is] = one(vs + (long)is * n);
+#pragma unroll
+    for (int ph = 0; ph <= NS; ++ph) A.vel[(long)ph * A.nfp + pt] = vel[ph];
+    if (A.ext[pt]) hn_d<NS>(g, K, F, ib, c, nxt[a]);
+    vg[c] = vel[0];
+#pragma unroll
+    for (int is = 0; is < NS; ++is) vs[(long)is * n + c] = vel[1 + is];
+  }
+}
+
+// boundary over a list of cells (one level of the schedule built by the
+// host): cells of one level read and write nothing another cell of the
+// level writes, and the levels run in order, so the result is the serial
+// loop's (bdry.f:77)
+template <int D, int NS>
+__global__ void __launch_bounds__(kBlock) k_boundary_list(Grid g, Consts K, Fields F, Ib ib, const int32_t *cells,
+                                                          int count) {
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= count) return;
+  const long c = cells[t];
+  const int32_t f = g.flag[c];
+  if (f == IMMB_CELL || f == FILLED_CELL_1 || f == FILLED_CELL_2) ib_boundary_cell<D, NS>(g, K, F, ib, c);
+  else boundary_cell<D, NS>(g, K, F, c);
+}
+
 // ======================================================================
 // fieldn, tilde.f:125-274: step-start copies, face momenta, viscosity
 template <int D, int NS>
@@ -526,6 +607,12 @@ __device__ __forceinline__ void momflux_cell(const Grid &g, const Consts &K, con
   gather_rnb<D>(x, uu, u);
   gather_rnb<D>(x, ww, w);
   if (D == 3) gather_rnb<D>(x, vv, v);
+  if (F.ibm && (fc == BL_CELL || fc == IMMB_CELL)) {
+    // velocities of faces inside the topography masked (tilde.f:1072-1118)
+    mask_rnb<D>(x, F.ibm, IBM_E, u);
+    mask_rnb<D>(x, F.ibm, IBM_T, w);
+    if (D == 3) mask_rnb<D>(x, F.ibm, IBM_N, v);
+  }
   gather_nb<D>(x, dd, dens);
   interp_x<D>(g, dens, sx, i);
   interp_z<D>(g, dens, sz, k);
@@ -669,7 +756,8 @@ __device__ __forceinline__ void inter_axis(const double *kpgv, double dt, const 
 // need every velocity component; the rest is done one axis at a time (the
 // axes share no accumulator), which keeps one axis's values live at once.
 template <int D, int NS, typename FT>
-__global__ void __launch_bounds__(kBlock, D == 2 ? 3 : 2) k_tilde(Grid g, Consts K, Fields F, double dt, int old_zero) {
+__global__ void __launch_bounds__(kBlock, D == 2 ? 3 : 2) k_tilde(Grid g, Consts K, Fields F, Ib ib, double dt,
+                                                                   int old_zero) {
   long c = cell_id();
   if (c >= g.n) return;
   if (!computed(g.flag[c])) return;
@@ -829,6 +917,38 @@ __global__ void __launch_bounds__(kBlock, D == 2 ? 3 : 2) k_tilde(Grid g, Consts
           rs[l - 2] = rs[l - 2] - CN(ls) * vsx[oll + c];
           rs[ll - 2] = rs[ll - 2] - CN(ls) * vsx[ol + c];
         }
+      }
+    }
+    // forcing on the immersed boundary, tilde.f:711-803: the explicit terms
+    // of a forcing point's component drive it to the point's pseudo-velocity
+    if (ib.num[ax] && g.flag[c] == IMMB_CELL && ib.num[ax][c] != 0) {
+      const IbAxis &A = ib.ax[ax];
+      const int pt = ib.num[ax][c] - 1;
+      const double *pr = F.f[PDAC_P];
+      double force = (ind * (d1 * rgp[c] + d0 * rgp[qn]) * A.vel[pt] - r0) / dt;
+      double epf = (d0 * ep[qn] + d1 * ep[c]) * ind;
+      double presn = -ind * 2.0 * epf * (pr[qn] - pr[c]);
+      double dragn = 0.0;
+      double drel[NS];
+#pragma unroll
+      for (int is = 0; is < NS; ++is) {
+        const long os = (long)is * n;
+        drel[is] = ax == 0 ? ((ug[c] - us[os + c]) + (ug[q.imjk] - us[os + q.imjk])) * 0.5
+                 : ax == 1 ? ((vg[c] - vs[os + c]) + (vg[q.ijmk] - vs[os + q.ijmk])) * 0.5
+                           : ((wg[c] - ws[os + c]) + (wg[q.ijkm] - ws[os + q.ijkm])) * 0.5;
+        dragn = dragn - kpgv[is] * drel[is];
+      }
+      force = force - presn - dragn;
+      r0 = r0 + dt * force;
+#pragma unroll
+      for (int is = 0; is < NS; ++is) {
+        const long os = (long)is * n;
+        force = (ind * (d1 * rlk[os + c] + d0 * rlk[os + qn]) * A.vel[(long)(1 + is) * A.nfp + pt] - rs[is]) / dt;
+        epf = (d0 * rlk[os + qn] + d1 * rlk[os + c]) * ind * K.inrl[is];
+        presn = -ind * 2.0 * epf * (pr[qn] - pr[c]);
+        dragn = kpgv[is] * drel[is];
+        force = force - presn - dragn;
+        rs[is] = rs[is] + dt * force;
       }
     }
     F.f[PDAC_RUG + ax][c] = r0;
