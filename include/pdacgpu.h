@@ -257,6 +257,10 @@ int pdac_profile_read(pdac_ctx *ctx, int kernel, int64_t *launches, double *tota
 /* cells visited by PDAC_K_SWEEP launches (not converged at visit start)
  * since the last pdac_profile_reset */
 int pdac_profile_visits(pdac_ctx *ctx, int64_t *visits);
+/* cells whose mass residual PDAC_K_SWEEP launches evaluated (the cells of a
+ * colour phase whose inputs changed since their last check) since the last
+ * pdac_profile_reset */
+int pdac_profile_checks(pdac_ctx *ctx, int64_t *checks);
 
 #ifdef __cplusplus
 }

@@ -52,6 +52,43 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "pdac_drive.x")
+
+
+def flang() -> str | None:
+    for cand in (os.environ.get("FC"), "/opt/rocm/llvm/bin/flang", "/opt/rocm/lib/llvm/bin/flang",
+                 shutil.which("flang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the iso_c_binding module (fortran/pdac_gpu.f90) and the
+    main_prog-shaped driver (fortran/pdac_drive.f90) against LIB with the
+    image's flang; returns the driver's path, or None without flang."""
+    fc = flang()
+    if fc is None:
+        return None
+    work = os.path.join(FORTRAN, "_build")
+    os.makedirs(work, exist_ok=True)
+    obj = os.path.join(work, "pdac_gpu.o")
+    tmp = DRIVER + ".tmp"
+    cmds = [[fc, "-O2", "-c", os.path.join(FORTRAN, "pdac_gpu.f90"), "-J", work, "-o", obj],
+            [fc, "-O2", "-J", work, os.path.join(FORTRAN, "pdac_drive.f90"), obj, "-L", HERE, "-lpdacgpu",
+             "-Wl,-rpath,$ORIGIN/..", "-o", tmp]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"flang failed with status {res.returncode}")
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def stale() -> bool:
     if not os.path.exists(LIB):
         return True
@@ -62,3 +99,4 @@ def stale() -> bool:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

@@ -25,7 +25,7 @@ EXPORTS = [
     "pdac_profile_enable", "pdac_profile_select", "pdac_profile_reset", "pdac_profile_read",
     "pdac_profile_visits", "pdac_initialize_fieldn", "pdac_explicit_iter",
     "pdac_ens_create", "pdac_ens_destroy", "pdac_ens_tune", "pdac_ens_set_tiles", "pdac_ens_step",
-    "pdac_explicit_step_noalloc", "pdac_set_ib",
+    "pdac_explicit_step_noalloc", "pdac_set_ib", "pdac_profile_checks",
 ]
 
 
@@ -46,6 +46,7 @@ def lib() -> ct.CDLL:
         L.pdac_profile_read.argtypes = [ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int64),
                                         ct.POINTER(ct.c_double)]
         L.pdac_profile_visits.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int64)]
+        L.pdac_profile_checks.argtypes = [ct.c_void_p, ct.POINTER(ct.c_int64)]
         L.pdac_sync.argtypes = [ct.c_void_p]
         L.pdac_ens_create.argtypes = [ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_void_p), ct.c_int]
         L.pdac_ens_destroy.argtypes = [ct.c_void_p]
