@@ -66,8 +66,11 @@ struct Fields {
   int32_t *ctr;                        // device counters (iter, ftem)
   unsigned long long *visits;          // work-list entries over all sweeps
   int32_t *wl;                         // work list of one colour phase
-  uint8_t *dirty;                      // cell must be checked again (iter)
-  int32_t *stamp;                      // colour phase whose start snap_* holds
+  // sweep flags (iter), one byte per cell in colour-major order (sweep_slot):
+  // sdirty, the cell must be checked again; srefresh, the cell of the other
+  // colour in the same pair changed in the previous phase and its snapshot
+  // is to be refreshed
+  uint8_t *sdirty, *srefresh;
   // immersed boundaries: per cell the face masks b_e..b_b and whether 1/vf
   // differs from 1 (IBM_* bits), and 1/vf; null without immersed boundaries
   const uint8_t *ibm;
