@@ -32,11 +32,13 @@ CONVERGING = [c for c in ALL if c != "mixed_2d_limiters"]
 def gpu(case, mode="persistent"):
     """GPU solver; mode selects how the red-black sweeps are launched: all in
     one persistent kernel, one launch per colour phase, or batches of such
-    launches replayed as HIP graphs.  "launches:N" also fixes the cells each
-    thread of the sweep kernel checks (N = 1..4), which the fixtures' small
-    meshes would otherwise leave at one; "launches:N:late" reads the sweep
+    launches replayed as HIP graphs.  "launches:N" also fixes the flag words
+    (4 cells each) each thread of the sweep kernel scans (N = 1..64), which
+    the fixtures' small meshes would otherwise leave at one; "launches:N:late" reads the sweep
     counters before the check's first reads, as on meshes too large for a
-    resident grid."""
+    resident grid; "launches:N:flush" visits the block's list of unconverged
+    cells as soon as it holds more than one, between the checks of the
+    thread's pairs of cells, as large meshes do when the list fills."""
     import os
 
     from pdac_amd.gpu import gpu_solver
@@ -44,13 +46,15 @@ def gpu(case, mode="persistent"):
     cpt, _, order = rest.partition(":")
     os.environ["PDAC_ITER_MODE"] = mode
     if cpt:
-        os.environ["PDAC_SWEEP_CPT"] = cpt
-    if order:
+        os.environ["PDAC_SWEEP_WORDS"] = cpt
+    if order == "flush":
+        os.environ["PDAC_SWEEP_FLUSH"] = "1"
+    elif order:
         os.environ["PDAC_SWEEP_EARLY"] = "0" if order == "late" else "1"
     try:
         return gpu_solver(case, device=0)
     finally:
-        for k in ("PDAC_ITER_MODE", "PDAC_SWEEP_CPT", "PDAC_SWEEP_EARLY"):
+        for k in ("PDAC_ITER_MODE", "PDAC_SWEEP_WORDS", "PDAC_SWEEP_EARLY", "PDAC_SWEEP_FLUSH"):
             os.environ.pop(k, None)
 
 
@@ -86,7 +90,7 @@ STAGES = [
 
 
 @pytest.mark.parametrize("mode", ["persistent", "launches", "graph", "launches:2", "launches:3", "launches:4",
-                                  "launches:3:late"])
+                                  "launches:3:late", "launches:7:flush", "launches:64"])
 @pytest.mark.parametrize("name", ALL)
 def test_stagewise(name, mode):
     print(f"[stagewise] {name} {mode}", flush=True)
@@ -134,7 +138,7 @@ def test_momentum_fluxes(name):
             assert rel(got, want) <= 1e-14, (comp, face, rel(got, want))
 
 
-@pytest.mark.parametrize("mode", ["persistent", "launches", "graph", "launches:2:late"])
+@pytest.mark.parametrize("mode", ["persistent", "launches", "graph", "launches:2:late", "launches:5:flush"])
 @pytest.mark.parametrize("name", ALL)
 def test_steps_match_checker(name, mode):
     print(f"[steps] {name} {mode}", flush=True)
@@ -176,3 +180,40 @@ def test_steps_match_reference(name, mode):
             o.reshape(-1, case.ntot)[:, ghost] = 0.0
             v.reshape(-1, case.ntot)[:, ghost] = 0.0
         assert rel(o, v) <= TOL_REF[k], (k, rel(o, v))
+
+
+@pytest.mark.gpu
+def test_sweep_launch_shapes_agree_on_a_mid_size_cone():
+    """The sweep kernel's scan in several launch shapes (flag words per
+    thread, so that threads scan several rounds of words; list visits between
+    checks; counters read before or after the first flags; graph replay) and
+    the persistent kernel give the same states bit for bit over three steps of
+    a 64 x 64 x 48 cone with immersed boundaries, a mesh large enough for
+    several hundred chunks of flags; the checker in the same order agrees to
+    the semi-implicit tolerance."""
+    from oracle.binding import cpu_solver
+    from pdac_amd import abi
+    from pdac_amd.cases import summit_cone_3d
+    case = summit_cone_3d(nx=64, ny=64, nz=48, height=375.0, sigma=375.0, rough=7.5, nsteps=1)
+    ref, ref_nits, ref_state = None, None, None
+    for mode in ["launches", "launches:12", "launches:12:flush", "launches:5:late", "graph", "persistent"]:
+        print(f"[shapes] {mode}", flush=True)
+        g = gpu(case, mode)
+        nits = [g.step(1) for _ in range(3)]
+        st = all_fields(g)
+        if ref is None:
+            ref_state = g.state()
+        g.close()
+        if ref is None:
+            ref, ref_nits = st, nits
+            continue
+        assert nits == ref_nits, (mode, nits, ref_nits)
+        for k, v in ref.items():
+            assert np.array_equal(st[k], v), (mode, k)
+    c = cpu_solver(case, iter_order=abi.ORDER_REDBLACK)
+    cn = [c.step(1) for _ in range(3)]
+    cs = c.state()
+    c.close()
+    assert cn == ref_nits
+    for k, v in cs.items():
+        assert rel(ref_state[k], v) <= 1e-10, (k, rel(ref_state[k], v))
