@@ -57,6 +57,7 @@ def test_structs_match_header_layout():
     ((4, 0, 40, 1, 8, 1, 2, 0), "job_type"),
     ((3, 0, 40, 2, 8, 1, 2, 0), "mesh"),
     ((2, 0, 40, 1, 8, 1, 2, 2), "precision"),
+    ((3, 0, 2048, 1024, 1024, 1, 2, 0), "2^31"),
 ])
 def test_create_rejects_bad_dims(lib, dims, msg):
     ctx = ct.c_void_p()
