@@ -49,7 +49,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # phase) on the default workload (FP32-flux form): rocprofv3 FETCH_SIZE and
 # WRITE_SIZE over its dispatches, each scaled by what the counter reads for a
 # copy of known size (--copy-calibration); None until measured
-SWEEP_TRAFFIC_PER_LAUNCH = 136.28e6
+SWEEP_TRAFFIC_PER_LAUNCH = 36.73e6
 # order of enum pdac_kernel (include/pdacgpu.h)
 KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
            "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill",
