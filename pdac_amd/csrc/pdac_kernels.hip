@@ -655,11 +655,12 @@ __device__ __forceinline__ void momflux_cell(const Grid &g, const Consts &K, con
   }
 }
 
-// At least three waves per SIMD (168 VGPRs, a few spilled): the kernel is
-// bound by the latency of its gathers (4.00 -> 3.72 ms on the 256x256x192
-// cone against two waves at 182 VGPRs).
+// FP32 fluxes: at least three waves per SIMD (168 VGPRs, a few spilled): the
+// kernel is bound by the latency of its gathers (4.00 -> 3.72 ms on the
+// 256x256x192 cone against two waves at 182 VGPRs).  FP64 fluxes need 238
+// VGPRs and keep two waves.
 template <int D, int NS, typename FT>
-__global__ void __launch_bounds__(kBlock, 3) k_momflux(Grid g, Consts K, Fields F) {
+__global__ void __launch_bounds__(kBlock, sizeof(FT) == 4 ? 3 : 2) k_momflux(Grid g, Consts K, Fields F) {
   long c = cell_id();
   if (c >= g.n) return;
   int fc = g.flag[c];

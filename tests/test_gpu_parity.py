@@ -33,12 +33,12 @@ def gpu(case, mode="persistent"):
     """GPU solver; mode selects how the red-black sweeps are launched: all in
     one persistent kernel, one launch per colour phase, or batches of such
     launches replayed as HIP graphs.  "launches:N" also fixes the flag words
-    (4 cells each) each thread of the sweep kernel scans (N = 1..64), which
+    (4 cells each) each thread of the sweep kernel scans (N = 1..24), which
     the fixtures' small meshes would otherwise leave at one; "launches:N:late" reads the sweep
     counters before the check's first reads, as on meshes too large for a
     resident grid; "launches:N:flush" visits the block's list of unconverged
-    cells as soon as it holds more than one, between the checks of the
-    thread's pairs of cells, as large meshes do when the list fills."""
+    cells as soon as it holds more than one, between passes of checks, as a
+    block does when its list could overflow."""
     import os
 
     from pdac_amd.gpu import gpu_solver
@@ -90,7 +90,7 @@ STAGES = [
 
 
 @pytest.mark.parametrize("mode", ["persistent", "launches", "graph", "launches:2", "launches:3", "launches:4",
-                                  "launches:3:late", "launches:7:flush", "launches:64"])
+                                  "launches:3:late", "launches:7:flush", "launches:24"])
 @pytest.mark.parametrize("name", ALL)
 def test_stagewise(name, mode):
     print(f"[stagewise] {name} {mode}", flush=True)
