@@ -28,6 +28,7 @@
 #include "pdac_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -808,3 +809,6 @@ void pdac_cpu_gauss_jordan(int np, const double *plim, double *a, double *b) {
 
 #include "pdac_oracle_step.inc"
 #include "pdac_oracle_explicit.inc"
+
+/* threads the parallel cell loops run on (OMP_NUM_THREADS) */
+int pdac_cpu_threads(void) { return omp_get_max_threads(); }

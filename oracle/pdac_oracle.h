@@ -65,4 +65,7 @@ int pdac_cpu_get_momflux(pdac_cpu_ctx *ctx, int comp, int face, double *out);
 #ifdef __cplusplus
 }
 #endif
+/* threads the order-independent cell loops run on (OpenMP) */
+int pdac_cpu_threads(void);
+
 #endif
