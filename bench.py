@@ -380,7 +380,7 @@ def main():
                          "one), or none")
     ap.add_argument("--forms", choices=["both", "fp64", "fp32flux"], default="both",
                     help="precision forms to run; with both, value is the FP32-flux form's")
-    ap.add_argument("--cpu-steps", type=int, default=12,
+    ap.add_argument("--cpu-steps", type=int, default=40,
                     help="steps of the CPU sample (0: skip the CPU baseline)")
     ap.add_argument("--column-steps", type=int, default=200,
                     help="steps of the 2D eruption column per form (0: skip it)")
