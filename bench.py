@@ -53,7 +53,8 @@ SWEEP_TRAFFIC_PER_LAUNCH = 36.73e6
 # order of enum pdac_kernel (include/pdacgpu.h)
 KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
            "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill",
-           "sweep_check", "iterate", "xinit", "xflux", "xrhs", "xchain", "xmass", "xsolve"]
+           "sweep_check", "iterate", "xinit", "xflux", "xrhs", "xchain", "xmass", "xsolve",
+           "stress"]
 
 
 def words(D, S, G):

@@ -87,6 +87,13 @@ typedef struct pdac_params {
   int32_t cstop;                    /* stop when iter does not converge */
   int32_t iter_order;               /* PDAC_ORDER_* */
   int32_t time_integration;         /* 0 semi-implicit, 1 explicit (prog.f:192-268) */
+  /* viscous stresses and heat conduction (&model, input.f:247-251): the gas
+   * stress tensor (viscg, visc.f:150-180) and conduction (hotc.f:12-84),
+   * the particle stress tensor (viscs, visc.f:182-291) and conduction, the
+   * Coulombic repulsive term (repulsive_model = 1, visc.f:232-289); the
+   * sub-grid turbulence models are off (iturb = iss = 0) */
+  int32_t gas_viscosity, part_viscosity, repulsive_model;
+  double mus[PDAC_MAX_SOLID];       /* particle viscosity (&particles viscosity, input.f:1100) */
 } pdac_params;
 
 /* Field identifiers for pdac_put/pdac_get.  ncomp is 1 unless noted
@@ -247,7 +254,7 @@ enum pdac_kernel {
   PDAC_K_GASFLUX, PDAC_K_SNAPSHOT, PDAC_K_SWEEP, PDAC_K_YGAS_FLUX, PDAC_K_YGAS_UPDATE,
   PDAC_K_HTILDE_FLUX, PDAC_K_HTILDE_UPDATE, PDAC_K_FTEM, PDAC_K_FILL, PDAC_K_SWEEP_CHECK,
   PDAC_K_ITERATE, PDAC_K_XINIT, PDAC_K_XFLUX, PDAC_K_XRHS, PDAC_K_XCHAIN, PDAC_K_XMASS,
-  PDAC_K_XSOLVE, PDAC_K_COUNT
+  PDAC_K_XSOLVE, PDAC_K_STRESS, PDAC_K_COUNT
 };
 int pdac_profile_enable(pdac_ctx *ctx, int enable);
 /* bit k of mask: time kernel k (default: all) */

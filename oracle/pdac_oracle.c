@@ -69,6 +69,8 @@ struct pdac_cpu_ctx {
   double *mf[3][3];
   /* enthalpy and species fluxes */
   double *egf[3], *esf[3], *yf[3];
+  /* viscous stress divergence along x, y, z: np planes (gas, then each class) */
+  double *vis[3];
   double *conv, *abeta, *appu_old[3];
   double *snap_rgp, *snap_rlk;
   struct xstate *xs; /* explicit Runge-Kutta state, allocated on first use */
@@ -140,6 +142,7 @@ int pdac_cpu_create(pdac_cpu_ctx **out, const pdac_dims *d) {
     s->esf[c] = (double *)calloc((size_t)s->n * s->ns, sizeof(double));
     s->yf[c] = (double *)calloc((size_t)s->n * s->ng, sizeof(double));
     s->appu_old[c] = (double *)calloc((size_t)s->n * s->na, sizeof(double));
+    s->vis[c] = (double *)calloc((size_t)s->n * s->np, sizeof(double));
   }
   s->conv = (double *)calloc((size_t)s->n, sizeof(double));
   s->abeta = (double *)calloc((size_t)s->n, sizeof(double));
@@ -168,7 +171,7 @@ int pdac_cpu_destroy(pdac_cpu_ctx *s) {
   for (int id = 0; id < PDAC_NFIELDS; ++id) free(s->f[id]);
   for (int c = 0; c < 3; ++c) {
     for (int fc = 0; fc < 3; ++fc) free(s->mf[c][fc]);
-    free(s->egf[c]); free(s->esf[c]); free(s->yf[c]); free(s->appu_old[c]);
+    free(s->egf[c]); free(s->esf[c]); free(s->yf[c]); free(s->appu_old[c]); free(s->vis[c]);
   }
   free(s->conv); free(s->abeta); free(s->snap_rgp); free(s->snap_rlk);
   free(s->dx); free(s->indx); free(s->r); free(s->rb); free(s->inr);

@@ -60,6 +60,8 @@ class Params(ct.Structure):
         ("pmodel", ct.c_int32), ("drag_model", ct.c_int32), ("icpc", ct.c_int32),
         ("ord", ct.c_int32), ("cstop", ct.c_int32), ("iter_order", ct.c_int32),
         ("time_integration", ct.c_int32),
+        ("gas_viscosity", ct.c_int32), ("part_viscosity", ct.c_int32),
+        ("repulsive_model", ct.c_int32), ("mus", ct.c_double * PDAC_MAX_SOLID),
     ]
 
 
@@ -93,6 +95,11 @@ def params_from_case(case, iter_order=ORDER_REDBLACK) -> Params:
     p.cstop = 1 if case.cstop else 0
     p.iter_order = iter_order
     p.time_integration = case.time_integration
+    p.gas_viscosity = 1 if case.gas_viscosity else 0
+    p.part_viscosity = 1 if case.part_viscosity else 0
+    p.repulsive_model = case.repulsive_model
+    for s in range(case.nsolid):
+        p.mus[s] = case.viscosity[s]  # cmus, input.f:1100
     return p
 
 

@@ -234,14 +234,18 @@ def summit_vent_3d(nx=256, ny=256, nz=192, d=10.0, nsolid=3, seed=1234, vent_hal
 
 
 def mixed_box(three_d=False, nx=10, ny=7, nz=9, nsolid=2, seed=7, itc=0,
-              gravz=-9.81, nsteps=2, dt=2.0e-4, clean_top=0, **kw) -> Case:
+              gravz=-9.81, nsteps=2, dt=2.0e-4, clean_top=0, cell=1.0, vmax=20.0,
+              prange=(0.9e5, 1.3e5), **kw) -> Case:
     """Small randomised domain for parity tests: non-uniform cells, several
     particle classes, seeded smooth random fields, walls of both kinds.
-    ``clean_top`` rows below the lid start without particles."""
+    ``clean_top`` rows below the lid start without particles.  Cell sizes
+    are ``cell`` times U[0.8, 1.25], velocities U[-vmax, vmax] and
+    pressures U[prange] before smoothing (cells of 0.1 mm make viscous
+    stresses and conduction as large as the fluxes)."""
     rng = np.random.default_rng(seed)
-    dx = rng.uniform(0.8, 1.25, nx)
-    dz = rng.uniform(0.8, 1.25, nz)
-    dy = rng.uniform(0.8, 1.25, ny) if three_d else None
+    dx = rng.uniform(0.8, 1.25, nx) * cell
+    dz = rng.uniform(0.8, 1.25, nz) * cell
+    dy = rng.uniform(0.8, 1.25, ny) * cell if three_d else None
     diam = list(rng.choice([30.0, 100.0, 250.0, 1000.0], size=nsolid, replace=False))
     dens = list(rng.uniform(1000.0, 2700.0, nsolid))
     case = Case(job_type="3D" if three_d else "2D", itc=0 if three_d else itc,
@@ -258,7 +262,7 @@ def mixed_box(three_d=False, nx=10, ny=7, nz=9, nsolid=2, seed=7, itc=0,
         for ax in range(3):
             f = 0.5 * f + 0.25 * (np.roll(f, 1, ax) + np.roll(f, -1, ax))
         return f.reshape(-1)
-    p = smooth(0.9e5, 1.3e5)
+    p = smooth(*prange)
     tg = smooth(300.0, 900.0)
     eps = np.stack([smooth(0.0, 0.02) for _ in range(nsolid)])
     if clean_top:
@@ -267,7 +271,7 @@ def mixed_box(three_d=False, nx=10, ny=7, nz=9, nsolid=2, seed=7, itc=0,
     ts = np.stack([tg + smooth(-50.0, 50.0) for _ in range(nsolid)])
     y0 = smooth(0.0, 1.0)
     ygc = np.stack([y0, 1.0 - y0])
-    vel = lambda: smooth(-20.0, 20.0)
+    vel = lambda: smooth(-vmax, vmax)
     kwv = dict(ug=vel(), wg=vel(), us=np.stack([vel() for _ in range(nsolid)]),
                ws=np.stack([vel() for _ in range(nsolid)]))
     if three_d:

@@ -39,6 +39,7 @@ struct Grid {
   const double *p_atm, *t_atm;         // 1..nz
   int nx, ny, nz;
   long n;
+  int itc;                             // 1: 2D cylindrical
 };
 
 struct Consts {
@@ -53,6 +54,10 @@ struct Consts {
   double visw[7][7];
   int gas_type[7];
   int ngas, muscl, lm, lv, inmax, maxout, pmodel, drag_model, icpc, ord;
+  // viscous stresses and conduction (gas_viscosity, part_viscosity,
+  // repulsive_model) and the particle viscosity and conductivity per class
+  int gas_visc, part_visc, repulsive;
+  double mus[kMaxS], kap[kMaxS];
 };
 
 // All device arrays of one context.
@@ -75,6 +80,9 @@ struct Fields {
   // differs from 1 (IBM_* bits), and 1/vf; null without immersed boundaries
   const uint8_t *ibm;
   const double *ivf;
+  // viscous stress divergence along x, y, z (ntot, P): gas, then each
+  // class; null when neither phase is viscous
+  double *vis[3];
 };
 
 // face masks of a cell (b_e_ .. b_b_, immb.f:56-58): bits of Fields::ibm

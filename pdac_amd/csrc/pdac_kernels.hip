@@ -837,7 +837,8 @@ __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F,
       double fx = (double)m0[c] - (double)m0[q.imjk];
       double fz = (double)m2[c] - (double)m2[q.ijkm];
       double fy = (D == 3) ? (double)m1[c] - (double)m1[q.ijmk] : 0.0;
-      t = 0.0;
+      // the viscous stress divergence (gvisx/y/z, tilde.f:487-506) opens the sum
+      t = F.vis[0] ? F.vis[ax][c] : 0.0;
       if (ax == 0) {
         t = t - indxp * 2.0 * fx * g.inrb[i];
         t = t + (1.0 - ag) * (dx[i + 1] * rgpn[c] + dx[i] * rgpn[q.e]) * indxp * K.gravx;
@@ -870,7 +871,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F,
       double epsn_f = 0.0;
       if (K.pmodel == 1) epsn_f = (d0 * rlkn[os + qn] + d1 * rlkn[os + c]) * ind * K.inrl[is];
       const double sink = 0.0;
-      t = 0.0;
+      t = F.vis[0] ? F.vis[ax][o + c] : 0.0;
       if (ax == 0) {
         t = t - indxp * 2.0 * fx * g.inrb[i];
         t = t + (1.0 - ag) * indxp * K.gravx * (dx[i + 1] * rlkn[os + c] + dx[i] * rlkn[os + q.e]);
@@ -975,6 +976,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F,
   }
 }
 
+#include "pdac_visc.hip.inc"
 #include "pdac_iter.hip.inc"
 #include "pdac_energy.hip.inc"
 #include "pdac_explicit.hip.inc"

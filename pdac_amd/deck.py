@@ -53,6 +53,11 @@ class Case:
     # model
     icpc: int = 2
     tref: float = 298.0
+    # viscous stresses and heat conduction (&model, input.f:247-251);
+    # the turbulence models stay off (iturb = iss = 0)
+    gas_viscosity: bool = False
+    part_viscosity: bool = False
+    repulsive_model: int = 0
     drag_model: int = 1
     pmodel: int = 2
     gravx: float = 0.0
@@ -286,7 +291,8 @@ class Case:
         lines.append("/")
         lines.append("&model")
         lines.append(f" icpc = {self.icpc}, tref = {d(self.tref)}, irex = 1,")
-        lines.append(" gas_viscosity = F, part_viscosity = F, iss = 0, iturb = 0,")
+        lines.append(f" gas_viscosity = {b(self.gas_viscosity)}, part_viscosity = {b(self.part_viscosity)},")
+        lines.append(f" iss = 0, iturb = 0, repulsive_model = {self.repulsive_model},")
         lines.append(f" drag_model = {self.drag_model}, pmodel = {self.pmodel},")
         lines.append(f" gravx = {d(self.gravx)}, gravy = {d(self.gravy)}, gravz = {d(self.gravz)},")
         lines.append(f" ngas = {len(self.gas_type)}, isink = 0")

@@ -9,7 +9,8 @@
 ! IN is a little-endian stream file: the dims (8 int32); the params
 ! component by component (12 doubles, dk rl phis cps kap plim of
 ! PDAC_MAX_SOLID doubles each, atm_ygc of PDAC_MAX_GAS doubles, gas_type of
-! PDAC_MAX_GAS int32, 12 int32); nsteps, rungekut, mode (int32); dt (double);
+! PDAC_MAX_GAS int32, 15 int32, mus of PDAC_MAX_SOLID doubles); nsteps,
+! rungekut, mode (int32); dt (double);
 ! immb (int32); dx(nx), dy(ny), dz(nz), p_atm(nz), t_atm(nz) (doubles);
 ! flag(ntot) (int32); when immb = 1 the immersed-boundary arrays of
 ! pdac_set_ib (nfp(3), fpi(7*sum nfp), fpd(3*sum nfp), coords(2nx+2ny+2nz),
@@ -31,7 +32,7 @@ PROGRAM pdac_drive
   TYPE(pdac_params) :: prm
   TYPE(C_PTR) :: ctx
   CHARACTER(LEN=1024) :: arg_in, arg_out
-  INTEGER(C_INT32_T) :: d8(8), pint(12), nsteps, rungekut, mode, immb, nput, nget, fid
+  INTEGER(C_INT32_T) :: d8(8), pint(15), nsteps, rungekut, mode, immb, nput, nget, fid
   INTEGER(C_INT32_T) :: nfp(3), nit, conv, fail, nit_last
   INTEGER(C_INT64_T) :: nit_total, cnt, ntot
   REAL(C_DOUBLE) :: pdbl(12), dt, dts, cfl
@@ -71,6 +72,8 @@ CONTAINS
     prm%maxout = pint(5); prm%pmodel = pint(6); prm%drag_model = pint(7); prm%icpc = pint(8)
     prm%ord = pint(9); prm%cstop = pint(10); prm%iter_order = pint(11)
     prm%time_integration = pint(12)
+    prm%gas_viscosity = pint(13); prm%part_viscosity = pint(14); prm%repulsive_model = pint(15)
+    READ(u) prm%mus
     READ(u) nsteps, rungekut, mode, dt, immb
     ntot = INT(dims%nx, C_INT64_T) * dims%ny * dims%nz
     ALLOCATE(dx(dims%nx), dy(dims%ny), dz(dims%nz), p_atm(dims%nz), t_atm(dims%nz), flag(ntot))

@@ -35,6 +35,8 @@ MODULE pdac_gpu
     INTEGER(C_INT32_T) :: gas_type(PDAC_MAX_GAS)
     INTEGER(C_INT32_T) :: muscl, lm, lv, inmax, maxout, pmodel, drag_model, icpc, ord, &
          cstop, iter_order, time_integration
+    INTEGER(C_INT32_T) :: gas_viscosity, part_viscosity, repulsive_model
+    REAL(C_DOUBLE) :: mus(PDAC_MAX_SOLID)
   END TYPE pdac_params
 
   ! field identifiers (enum pdac_field)

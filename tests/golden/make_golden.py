@@ -22,6 +22,16 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from pdac_amd.cases import eruption_column, mixed_box, shock_tube, summit_cone_3d  # noqa: E402
 from tests.golden.refrun import available, run_reference  # noqa: E402
 
+def viscous_box(nsteps=4, **kw):
+    """mixed_box on cells of about 0.1 mm (pressures within 10 Pa,
+    velocities within 1 m/s, dt 0.2 us) with the viscous stresses, the
+    repulsive term and conduction switched on."""
+    opts = dict(nsteps=nsteps, bottom=3, cell=1.0e-4, vmax=1.0, prange=(1.0e5, 1.0001e5),
+                dt=2.0e-7, gas_viscosity=True, part_viscosity=True, repulsive_model=1)
+    opts.update(kw)
+    return mixed_box(**opts)
+
+
 def explicit(case):
     """The same case advanced by the explicit Runge-Kutta solver
     (time_integration = 1, explicit_solver.f)."""
@@ -95,6 +105,16 @@ CASES = {
     "explicit_mixed_2d_icpc0": lambda: explicit(mixed_box(nsteps=2, bottom=3, icpc=0, seed=17)),
     "explicit_mixed_2d_zerograd": lambda: explicit(mixed_box(nsteps=2, bottom=3, east=6, west=6,
                                                             seed=13)),
+    # viscous stresses (gas and particles, Coulombic repulsion) and heat
+    # conduction on cells of 0.1 mm, where they are as large as the fluxes
+    "visc_2d": lambda: viscous_box(),
+    "visc_2d_gas": lambda: viscous_box(part_viscosity=False, seed=29),
+    "visc_2d_cyl_freeio": lambda: viscous_box(itc=1, west=2, east=4, top=2, stratification=True),
+    "visc_3d": lambda: viscous_box(three_d=True, nsolid=3, west=3, east=4, south=4, north=3, top=2),
+    "explicit_visc_2d_cyl": lambda: explicit(viscous_box(itc=1, west=2, east=4, top=2,
+                                                         stratification=True, repulsive_model=0)),
+    "explicit_visc_3d": lambda: explicit(viscous_box(three_d=True, nsolid=3, west=3, east=4,
+                                                     south=4, north=3, top=2, nsteps=3)),
 }
 
 # Fixtures with a free in/outflow or zero-gradient lid: the reference sets

@@ -120,3 +120,17 @@ def test_ensemble_3d_members():
     e.step(2)
     assert_same(states(batch), states(solo), "3D")
     e.close()
+
+
+def test_ensemble_refuses_viscous_members():
+    """The ensemble kernels carry no viscous stresses or conduction: a
+    member with gas_viscosity set is refused with an error, not stepped
+    without them."""
+    from pdac_amd.cases import mixed_box
+    from pdac_amd.gpu import Ensemble, gpu_solver
+    ms = members_2d()
+    c = mixed_box(nsteps=1, bottom=3, gas_viscosity=True)
+    c.time_integration = 1
+    ms.append(gpu_solver(c, device=0))
+    with pytest.raises(Exception, match="viscous"):
+        Ensemble(ms)

@@ -158,6 +158,13 @@ def test_steps_match_checker(name, mode):
 TOL_REF = {"p": 1e-7, "rgp": 2e-7, "rog": 2e-7, "tg": 2e-7, "sieg": 2e-7, "rlk": 1e-7,
            "ygc": 1e-7, "sies": 1e-6, "ts": 1e-6, "ug": 3e-6, "vg": 3e-6, "wg": 3e-6,
            "us": 3e-6, "vs": 3e-6, "ws": 3e-6}
+# the viscous fixtures' cells of 0.1 mm and dt of 0.2 us make the pressure -
+# velocity coupling nearly incompressible (dt^2/dx^2 far above eps/c^2): a
+# mass residual within delg * rgp then moves the pressure by up to
+# delg * rho * dx^2 / dt^2 and the velocities by up to delg * dx / dt = 5e-6
+# m/s of about 1 m/s, so the two cell orders part by that much (the CPU
+# checker in red-black order misses these fixtures by up to 1.2e-5 in ug)
+TOL_REF_STIFF = dict(TOL_REF, ug=2e-5, vg=2e-5, wg=2e-5, us=2e-5, vs=2e-5, ws=2e-5)
 
 
 @pytest.mark.parametrize("mode", ["launches", "graph", "launches:2:late", "persistent"])
@@ -179,7 +186,8 @@ def test_steps_match_reference(name, mode):
             o, v = o.copy(), v.copy()
             o.reshape(-1, case.ntot)[:, ghost] = 0.0
             v.reshape(-1, case.ntot)[:, ghost] = 0.0
-        assert rel(o, v) <= TOL_REF[k], (k, rel(o, v))
+        tol = TOL_REF_STIFF if name.startswith("visc_") else TOL_REF
+        assert rel(o, v) <= tol[k], (k, rel(o, v))
 
 
 @pytest.mark.gpu

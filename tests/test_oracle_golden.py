@@ -93,3 +93,30 @@ def test_explicit_fixtures_pin_the_unset_mass_flux_neighbours():
     reference build reads there."""
     worst = max(_explicit_run(n, stale_zero=0) for n in EXPLICIT)
     assert worst > 1e-8
+
+
+VISCOUS = sorted(n for n in CASES if "visc" in n)
+
+
+@pytest.mark.parametrize("term", ["gas_viscosity", "part_viscosity", "repulsive_model"])
+def test_viscous_fixtures_pin_each_term(term):
+    """With one of the gas stresses and conduction, the particle stresses
+    and conduction or the Coulombic repulsive term left out of the oracle's
+    run, the viscous fixtures that have it on must be missed (the oracle
+    matches them bit for bit; the repulsive term is the smallest, about
+    5e-6 of the particle velocities over four steps): they pin each of the
+    three."""
+    worst, used = 0.0, 0
+    for name in VISCOUS:
+        case, ref, _ = load(name)
+        if not getattr(case, term):
+            continue
+        setattr(case, term, type(getattr(case, term))(0))
+        s = cpu_solver(case, iter_order=abi.ORDER_LEXICOGRAPHIC)
+        s.step(case.nsteps)
+        out = s.state()
+        used += 1
+        worst = max(worst, max(np.max(np.abs(out[k].reshape(v.shape) - v)) / max(np.max(np.abs(v)), 1e-300)
+                               for k, v in ref.items()))
+    assert used >= 2
+    assert worst > 1e-6, worst
