@@ -90,10 +90,16 @@ typedef struct pdac_params {
   /* viscous stresses and heat conduction (&model, input.f:247-251): the gas
    * stress tensor (viscg, visc.f:150-180) and conduction (hotc.f:12-84),
    * the particle stress tensor (viscs, visc.f:182-291) and conduction, the
-   * Coulombic repulsive term (repulsive_model = 1, visc.f:232-289); the
-   * sub-grid turbulence models are off (iturb = iss = 0) */
+   * Coulombic repulsive term (repulsive_model = 1, visc.f:232-289) */
   int32_t gas_viscosity, part_viscosity, repulsive_model;
   double mus[PDAC_MAX_SOLID];       /* particle viscosity (&particles viscosity, input.f:1100) */
+  /* gas sub-grid turbulence (&model iturb, cmut, input.f:252-256): 0 off,
+   * 1 the classical Smagorinsky model (modturbo = 1; sgsg, turbo.f:145-383,
+   * with the length scale of turbulence_setup, turbo.f:44-118); the
+   * roughness (iturb = 2), dynamic (modturbo = 2) and particle (iss = 1)
+   * models are not built */
+  double cmut;
+  int32_t iturb;
 } pdac_params;
 
 /* Field identifiers for pdac_put/pdac_get.  ncomp is 1 unless noted
@@ -134,6 +140,12 @@ int pdac_set_grid(pdac_ctx *ctx, const double *dx, const double *dy,
                   const double *dz, const int32_t *flag,
                   const double *p_atm, const double *t_atm);
 int pdac_set_params(pdac_ctx *ctx, const pdac_params *prm);
+
+/* sgsg (turbo.f:145-383), called by main_prog after fieldn or
+ * initialize_fieldn when iturb >= 1 (prog.f:207): the Smagorinsky turbulent
+ * viscosity mugt = rog * smag * |S| and conductivity kapgt = mugt * cg / 0.5
+ * of every computed cell.  A no-op when iturb = 0. */
+int pdac_sgsg(pdac_ctx *ctx);
 
 /* Immersed boundaries (immb = 1, 3D, semi-implicit): what the reference's
  * set-up leaves in MODULE immersed_boundaries (immb.f:33-60): the forcing

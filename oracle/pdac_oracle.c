@@ -71,6 +71,9 @@ struct pdac_cpu_ctx {
   double *egf[3], *esf[3], *yf[3];
   /* viscous stress divergence along x, y, z: np planes (gas, then each class) */
   double *vis[3];
+  /* gas viscosity and conductivity with the turbulent parts (mugt, kapgt of
+   * MODULE turbulence_model, turbo.f:35-40: zero until sgsg sets them) */
+  double *mugt, *kapgt;
   double *conv, *abeta, *appu_old[3];
   double *snap_rgp, *snap_rlk;
   struct xstate *xs; /* explicit Runge-Kutta state, allocated on first use */
@@ -145,6 +148,8 @@ int pdac_cpu_create(pdac_cpu_ctx **out, const pdac_dims *d) {
     s->vis[c] = (double *)calloc((size_t)s->n * s->np, sizeof(double));
   }
   s->conv = (double *)calloc((size_t)s->n, sizeof(double));
+  s->mugt = (double *)calloc((size_t)s->n, sizeof(double));
+  s->kapgt = (double *)calloc((size_t)s->n, sizeof(double));
   s->abeta = (double *)calloc((size_t)s->n, sizeof(double));
   s->snap_rgp = (double *)calloc((size_t)s->n, sizeof(double));
   s->snap_rlk = (double *)calloc((size_t)s->n * s->ns, sizeof(double));
@@ -173,7 +178,7 @@ int pdac_cpu_destroy(pdac_cpu_ctx *s) {
     for (int fc = 0; fc < 3; ++fc) free(s->mf[c][fc]);
     free(s->egf[c]); free(s->esf[c]); free(s->yf[c]); free(s->appu_old[c]); free(s->vis[c]);
   }
-  free(s->conv); free(s->abeta); free(s->snap_rgp); free(s->snap_rlk);
+  free(s->conv); free(s->abeta); free(s->mugt); free(s->kapgt); free(s->snap_rgp); free(s->snap_rlk);
   free(s->dx); free(s->indx); free(s->r); free(s->rb); free(s->inr);
   free(s->inrb); free(s->dy); free(s->indy); free(s->dz); free(s->indz);
   free(s->p_atm); free(s->t_atm); free(s->flag);

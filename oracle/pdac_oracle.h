@@ -34,6 +34,7 @@ int pdac_cpu_field_ncomp(const pdac_cpu_ctx *ctx, int field);
 int pdac_cpu_cnvert(pdac_cpu_ctx *ctx);
 int pdac_cpu_boundary(pdac_cpu_ctx *ctx);
 int pdac_cpu_fieldn(pdac_cpu_ctx *ctx);
+int pdac_cpu_sgsg(pdac_cpu_ctx *ctx);
 int pdac_cpu_begin_step(pdac_cpu_ctx *ctx);
 int pdac_cpu_tilde(pdac_cpu_ctx *ctx, double dt);
 int pdac_cpu_iter(pdac_cpu_ctx *ctx, double dt, int32_t *nit, int32_t *converged);

@@ -62,6 +62,7 @@ class Params(ct.Structure):
         ("time_integration", ct.c_int32),
         ("gas_viscosity", ct.c_int32), ("part_viscosity", ct.c_int32),
         ("repulsive_model", ct.c_int32), ("mus", ct.c_double * PDAC_MAX_SOLID),
+        ("cmut", ct.c_double), ("iturb", ct.c_int32),
     ]
 
 
@@ -100,6 +101,7 @@ def params_from_case(case, iter_order=ORDER_REDBLACK) -> Params:
     p.repulsive_model = case.repulsive_model
     for s in range(case.nsolid):
         p.mus[s] = case.viscosity[s]  # cmus, input.f:1100
+    p.cmut, p.iturb = case.cmut, case.iturb
     return p
 
 

@@ -58,6 +58,7 @@ struct Consts {
   // repulsive_model) and the particle viscosity and conductivity per class
   int gas_visc, part_visc, repulsive;
   double mus[kMaxS], kap[kMaxS];
+  int iturb;  // 1: Smagorinsky sub-grid viscosity and conductivity (mugt, kapgt)
 };
 
 // All device arrays of one context.
@@ -83,6 +84,11 @@ struct Fields {
   // viscous stress divergence along x, y, z (ntot, P): gas, then each
   // class; null when neither phase is viscous
   double *vis[3];
+  // with the sub-grid model: the gas viscosity and conductivity with their
+  // turbulent parts (mugt, kapgt) and the squared Smagorinsky length of each
+  // cell (smag); null without it
+  double *mugt, *kapgt;
+  const double *smag;
 };
 
 // face masks of a cell (b_e_ .. b_b_, immb.f:56-58): bits of Fields::ibm

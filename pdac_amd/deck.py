@@ -58,6 +58,9 @@ class Case:
     gas_viscosity: bool = False
     part_viscosity: bool = False
     repulsive_model: int = 0
+    # gas sub-grid turbulence: 0 off, 1 classical Smagorinsky (modturbo = 1)
+    iturb: int = 0
+    cmut: float = 0.1
     drag_model: int = 1
     pmodel: int = 2
     gravx: float = 0.0
@@ -292,7 +295,8 @@ class Case:
         lines.append("&model")
         lines.append(f" icpc = {self.icpc}, tref = {d(self.tref)}, irex = 1,")
         lines.append(f" gas_viscosity = {b(self.gas_viscosity)}, part_viscosity = {b(self.part_viscosity)},")
-        lines.append(f" iss = 0, iturb = 0, repulsive_model = {self.repulsive_model},")
+        lines.append(f" iss = 0, iturb = {self.iturb}, modturbo = 1, cmut = {d(self.cmut)},")
+        lines.append(f" repulsive_model = {self.repulsive_model},")
         lines.append(f" drag_model = {self.drag_model}, pmodel = {self.pmodel},")
         lines.append(f" gravx = {d(self.gravx)}, gravy = {d(self.gravy)}, gravz = {d(self.gravz)},")
         lines.append(f" ngas = {len(self.gas_type)}, isink = 0")

@@ -9,7 +9,7 @@
 ! IN is a little-endian stream file: the dims (8 int32); the params
 ! component by component (12 doubles, dk rl phis cps kap plim of
 ! PDAC_MAX_SOLID doubles each, atm_ygc of PDAC_MAX_GAS doubles, gas_type of
-! PDAC_MAX_GAS int32, 15 int32, mus of PDAC_MAX_SOLID doubles); nsteps,
+! PDAC_MAX_GAS int32, 16 int32, mus of PDAC_MAX_SOLID doubles, cmut); nsteps,
 ! rungekut, mode (int32); dt (double);
 ! immb (int32); dx(nx), dy(ny), dz(nz), p_atm(nz), t_atm(nz) (doubles);
 ! flag(ntot) (int32); when immb = 1 the immersed-boundary arrays of
@@ -17,9 +17,9 @@
 ! bmask(6*ntot), ivf(ntot)); nput, then per field id (int32), count
 ! (int64), values; nget, then the ids of the fields to write back.
 ! mode 0 calls the subroutines one by one in main_prog's order (boundary,
-! fieldn, allocate_fluxes, and per Runge-Kutta stage with dt/(rungekut+1-rk)
+! fieldn, sgsg, allocate_fluxes, and per Runge-Kutta stage with dt/(rungekut+1-rk)
 ! tilde, iter, ygas, htilde, ftem; for time_integration = 1 boundary,
-! initialize_fieldn and per stage explicit_iter, ygas); mode 1 calls
+! initialize_fieldn, sgsg and per stage explicit_iter, ygas); mode 1 calls
 ! pdac_step for all steps at once.
 ! OUT: status, nit_last (int32), nit_total (int64), then per field written
 ! back its id (int32), count (int64) and values.
@@ -32,7 +32,7 @@ PROGRAM pdac_drive
   TYPE(pdac_params) :: prm
   TYPE(C_PTR) :: ctx
   CHARACTER(LEN=1024) :: arg_in, arg_out
-  INTEGER(C_INT32_T) :: d8(8), pint(15), nsteps, rungekut, mode, immb, nput, nget, fid
+  INTEGER(C_INT32_T) :: d8(8), pint(16), nsteps, rungekut, mode, immb, nput, nget, fid
   INTEGER(C_INT32_T) :: nfp(3), nit, conv, fail, nit_last
   INTEGER(C_INT64_T) :: nit_total, cnt, ntot
   REAL(C_DOUBLE) :: pdbl(12), dt, dts, cfl
@@ -73,7 +73,8 @@ CONTAINS
     prm%ord = pint(9); prm%cstop = pint(10); prm%iter_order = pint(11)
     prm%time_integration = pint(12)
     prm%gas_viscosity = pint(13); prm%part_viscosity = pint(14); prm%repulsive_model = pint(15)
-    READ(u) prm%mus
+    prm%iturb = pint(16)
+    READ(u) prm%mus, prm%cmut
     READ(u) nsteps, rungekut, mode, dt, immb
     ntot = INT(dims%nx, C_INT64_T) * dims%ny * dims%nz
     ALLOCATE(dx(dims%nx), dy(dims%ny), dz(dims%nz), p_atm(dims%nz), t_atm(dims%nz), flag(ntot))
@@ -122,6 +123,8 @@ CONTAINS
         IF (prm%time_integration == 1) THEN
           rc = pdac_initialize_fieldn(ctx, dt)
           CALL check('initialize_fieldn')
+          rc = pdac_sgsg(ctx)
+          CALL check('sgsg')
           DO rk = 1, rungekut
             dts = dt / DBLE(rungekut + 1 - rk)
             rc = pdac_explicit_iter(ctx, dts, fail, cfl)
@@ -132,6 +135,8 @@ CONTAINS
         ELSE
           rc = pdac_fieldn(ctx)
           CALL check('fieldn')
+          rc = pdac_sgsg(ctx)
+          CALL check('sgsg')
           rc = pdac_begin_step(ctx)
           CALL check('allocate_fluxes')
           DO rk = 1, rungekut

@@ -37,6 +37,8 @@ MODULE pdac_gpu
          cstop, iter_order, time_integration
     INTEGER(C_INT32_T) :: gas_viscosity, part_viscosity, repulsive_model
     REAL(C_DOUBLE) :: mus(PDAC_MAX_SOLID)
+    REAL(C_DOUBLE) :: cmut
+    INTEGER(C_INT32_T) :: iturb
   END TYPE pdac_params
 
   ! field identifiers (enum pdac_field)
@@ -116,6 +118,11 @@ MODULE pdac_gpu
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx
     END FUNCTION pdac_fieldn
+
+    INTEGER(C_INT) FUNCTION pdac_sgsg(ctx) BIND(C, name='pdac_sgsg')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION pdac_sgsg
 
     INTEGER(C_INT) FUNCTION pdac_begin_step(ctx) BIND(C, name='pdac_begin_step')
       IMPORT :: C_INT, C_PTR

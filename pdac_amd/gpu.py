@@ -25,7 +25,7 @@ EXPORTS = [
     "pdac_profile_enable", "pdac_profile_select", "pdac_profile_reset", "pdac_profile_read",
     "pdac_profile_visits", "pdac_initialize_fieldn", "pdac_explicit_iter",
     "pdac_ens_create", "pdac_ens_destroy", "pdac_ens_tune", "pdac_ens_set_tiles", "pdac_ens_step",
-    "pdac_explicit_step_noalloc", "pdac_set_ib", "pdac_profile_checks",
+    "pdac_explicit_step_noalloc", "pdac_set_ib", "pdac_profile_checks", "pdac_sgsg",
 ]
 
 

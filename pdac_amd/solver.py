@@ -34,7 +34,7 @@ def bind(lib: ct.CDLL, prefix: str):
         "destroy": [P], "set_grid": [P, dp, dp, dp, ip, dp, dp],
         "set_params": [P, ct.POINTER(abi.Params)],
         "put": [P, ct.c_int, dp, ct.c_int64], "get": [P, ct.c_int, dp, ct.c_int64],
-        "field_ncomp": [P, ct.c_int], "cnvert": [P], "boundary": [P], "fieldn": [P],
+        "field_ncomp": [P, ct.c_int], "cnvert": [P], "boundary": [P], "fieldn": [P], "sgsg": [P],
         "begin_step": [P], "tilde": [P, ct.c_double],
         "iter": [P, ct.c_double, ip, ip], "ygas": [P, ct.c_double],
         "htilde": [P, ct.c_double], "ftem": [P, ct.c_double, ip],
@@ -187,6 +187,9 @@ class Solver:
     def fieldn(self):
         self._check("fieldn", self.fn["fieldn"](self.ctx))
 
+    def sgsg(self):
+        self._check("sgsg", self.fn["sgsg"](self.ctx))
+
     def begin_step(self):
         self._check("begin_step", self.fn["begin_step"](self.ctx))
 
@@ -240,6 +243,7 @@ class Solver:
         if self.case.time_integration == 1:
             self.boundary()
             self.initialize_fieldn(dt)
+            self.sgsg()
             for rk in range(1, rk_n + 1):
                 dts = dt / (rk_n + 1 - rk)
                 self.explicit_iter(dts)
@@ -248,6 +252,7 @@ class Solver:
             return 0
         self.boundary()
         self.fieldn()
+        self.sgsg()
         self.begin_step()
         nit = 0
         for rk in range(1, rk_n + 1):

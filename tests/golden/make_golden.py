@@ -115,6 +115,18 @@ CASES = {
                                                          stratification=True, repulsive_model=0)),
     "explicit_visc_3d": lambda: explicit(viscous_box(three_d=True, nsolid=3, west=3, east=4,
                                                      south=4, north=3, top=2, nsteps=3)),
+    # the classical Smagorinsky sub-grid model (iturb = 1) with the viscous
+    # terms; cmut large enough that the turbulent viscosity outweighs mug
+    "visc_2d_sgs": lambda: viscous_box(iturb=1, cmut=0.5, seed=31),
+    "visc_3d_sgs": lambda: viscous_box(three_d=True, nsolid=2, west=3, east=4, south=4, north=3,
+                                       top=2, iturb=1, cmut=0.5, seed=37, ny=6),
+    "visc_2d_sgs_rk2": lambda: viscous_box(iturb=1, cmut=0.5, seed=41, rungekut=2, itc=1, west=2,
+                                           east=4, top=2),
+    "explicit_visc_2d_sgs": lambda: explicit(viscous_box(iturb=1, cmut=0.5, seed=43, itc=1, west=2,
+                                                         east=4, top=2)),
+    "explicit_visc_3d_sgs": lambda: explicit(viscous_box(three_d=True, nsolid=2, west=3, east=4,
+                                                         south=4, north=3, top=2, iturb=1, cmut=0.5,
+                                                         seed=47, ny=6, nsteps=3)),
 }
 
 # Fixtures with a free in/outflow or zero-gradient lid: the reference sets

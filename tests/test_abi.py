@@ -48,7 +48,7 @@ def test_structs_match_header_layout():
     # + 7 int32 + 12 int32
     assert abi.PDAC_MAX_SOLID == 4
     assert ct.sizeof(abi.Dims) == 32
-    assert ct.sizeof(abi.Params) == 8 * (12 + 6 * 4 + 7) + 4 * (7 + 12 + 3) + 8 * 4
+    assert ct.sizeof(abi.Params) == 8 * (12 + 6 * 4 + 7) + 4 * (7 + 12 + 3) + 8 * 4 + 8 + 4 + 4
 
 
 @pytest.mark.parametrize("dims,msg", [

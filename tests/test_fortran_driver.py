@@ -100,8 +100,8 @@ def write_domain(path, case, nsteps, mode, dt=None, rungekut=None, precision=abi
     parts.append(i32(list(p.gas_type)))
     parts.append(i32([p.muscl, p.lm, p.lv, p.inmax, p.maxout, p.pmodel, p.drag_model, p.icpc,
                       p.ord, p.cstop, p.iter_order, p.time_integration, p.gas_viscosity,
-                      p.part_viscosity, p.repulsive_model]))
-    parts.append(f64(list(p.mus)))
+                      p.part_viscosity, p.repulsive_model, p.iturb]))
+    parts.append(f64(list(p.mus) + [p.cmut]))
     parts.append(i32([nsteps, case.rungekut if rungekut is None else rungekut, mode]))
     parts.append(f64([case.dt if dt is None else dt]))
     immb = int(getattr(case, "immb", 0) or 0)

@@ -98,14 +98,14 @@ def test_explicit_fixtures_pin_the_unset_mass_flux_neighbours():
 VISCOUS = sorted(n for n in CASES if "visc" in n)
 
 
-@pytest.mark.parametrize("term", ["gas_viscosity", "part_viscosity", "repulsive_model"])
+@pytest.mark.parametrize("term", ["gas_viscosity", "part_viscosity", "repulsive_model", "iturb"])
 def test_viscous_fixtures_pin_each_term(term):
     """With one of the gas stresses and conduction, the particle stresses
-    and conduction or the Coulombic repulsive term left out of the oracle's
-    run, the viscous fixtures that have it on must be missed (the oracle
-    matches them bit for bit; the repulsive term is the smallest, about
-    5e-6 of the particle velocities over four steps): they pin each of the
-    three."""
+    and conduction, the Coulombic repulsive term or the Smagorinsky model
+    left out of the oracle's run, the viscous fixtures that have it on must
+    be missed (the oracle matches them bit for bit; the repulsive term is the
+    smallest, about 5e-6 of the particle velocities over four steps): they
+    pin each of the four."""
     worst, used = 0.0, 0
     for name in VISCOUS:
         case, ref, _ = load(name)
