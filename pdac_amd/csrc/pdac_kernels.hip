@@ -774,7 +774,8 @@ __device__ __forceinline__ void inter_axis(const double *kpgv, double dt, const 
 // the threads, each with a third of the axis work and fewer registers live
 // (three waves per SIMD), keep more of them in flight: 5.96 -> 4.53 ms on the
 // 256x256x192 cone.
-template <int D, int NS, typename FT>
+// (VISC: with the viscous stress divergence of k_stress)
+template <int D, int NS, typename FT, bool VISC>
 __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F, Ib ib, double dt, int old_zero) {
   int part;
   long c = cell_part_id(D == 2 ? 2 : 3, part);
@@ -838,7 +839,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F,
       double fz = (double)m2[c] - (double)m2[q.ijkm];
       double fy = (D == 3) ? (double)m1[c] - (double)m1[q.ijmk] : 0.0;
       // the viscous stress divergence (gvisx/y/z, tilde.f:487-506) opens the sum
-      t = F.vis[0] ? F.vis[ax][c] : 0.0;
+      t = VISC ? F.vis[ax][c] : 0.0;
       if (ax == 0) {
         t = t - indxp * 2.0 * fx * g.inrb[i];
         t = t + (1.0 - ag) * (dx[i + 1] * rgpn[c] + dx[i] * rgpn[q.e]) * indxp * K.gravx;
@@ -871,7 +872,7 @@ __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F,
       double epsn_f = 0.0;
       if (K.pmodel == 1) epsn_f = (d0 * rlkn[os + qn] + d1 * rlkn[os + c]) * ind * K.inrl[is];
       const double sink = 0.0;
-      t = F.vis[0] ? F.vis[ax][o + c] : 0.0;
+      t = VISC ? F.vis[ax][o + c] : 0.0;
       if (ax == 0) {
         t = t - indxp * 2.0 * fx * g.inrb[i];
         t = t + (1.0 - ag) * indxp * K.gravx * (dx[i + 1] * rlkn[os + c] + dx[i] * rlkn[os + q.e]);
