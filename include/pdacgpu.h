@@ -147,6 +147,29 @@ int pdac_set_params(pdac_ctx *ctx, const pdac_params *prm);
  * of every computed cell.  A no-op when iturb = 0. */
 int pdac_sgsg(pdac_ctx *ctx);
 
+/* Output and restart files without a host synchronisation in the step loop
+ * (outp, outp.f:194-384, at every nprint-th step; taperd, io_restart.f:31-70,
+ * at every ndump-th): pdac_snapshot_async packs the arrays of one file on the
+ * device and queues their download into a staging buffer of the library
+ * (device and pinned host memory of pdac_snapshot_size bytes, allocated by
+ * the first call of each kind), then returns; the steps queued after it
+ * overlap the download.  pdac_snapshot_fetch waits for that download and
+ * copies the arrays to host (bytes must be the size).  A second snapshot of
+ * a kind before its fetch replaces the first.  Array order and precision:
+ *   PDAC_SNAP_OUTPUT   float: p, ug, [vg], wg, tg, xgc(1..ngas), then per
+ *                      class rlk/rl, us, [vs], ws, ts (the records of an
+ *                      unformatted output.nnnn after its time record)
+ *   PDAC_SNAP_RESTART  double: p, ug, [vg], wg, sieg, rlk(1..nsolid), per
+ *                      class us, [vs], ws, sies(1..nsolid), ygc(1..ngas),
+ *                      rgp, rog, tg, ts(1..nsolid) (the records of pdac.res
+ *                      after its two header records)
+ * each array ntot values; [vg], [vs] on 3D meshes only. */
+#define PDAC_SNAP_OUTPUT 0
+#define PDAC_SNAP_RESTART 1
+int pdac_snapshot_size(pdac_ctx *ctx, int kind, int64_t *bytes);
+int pdac_snapshot_async(pdac_ctx *ctx, int kind);
+int pdac_snapshot_fetch(pdac_ctx *ctx, int kind, void *host, int64_t bytes);
+
 /* Immersed boundaries (immb = 1, 3D, semi-implicit): what the reference's
  * set-up leaves in MODULE immersed_boundaries (immb.f:33-60): the forcing
  * points of the x, y and z velocity components (TYPE forcing_point,

@@ -17,6 +17,8 @@ ORDER_STAGE = 2
 ST_ITER_NOT_CONVERGED = 1
 ST_EOS_FAILURE = 2
 ST_BAD_ARGUMENT = 3
+SNAP_OUTPUT = 0   # PDAC_SNAP_OUTPUT
+SNAP_RESTART = 1  # PDAC_SNAP_RESTART
 
 FIELDS = [
     "P", "EP", "RGP", "ROG", "TG", "SIEG", "UG", "VG", "WG",

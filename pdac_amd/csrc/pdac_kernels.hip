@@ -982,4 +982,5 @@ __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F,
 #include "pdac_energy.hip.inc"
 #include "pdac_explicit.hip.inc"
 #include "pdac_ensemble.hip.inc"
+#include "pdac_io.hip.inc"
 #include "pdac_api.hip.inc"

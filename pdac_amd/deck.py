@@ -92,6 +92,9 @@ class Case:
     nsteps: int = 1
     time: float = 0.0
     cstop: bool = False
+    # output files every tpr_steps steps (tpr = tpr_steps * dt); 0: only the
+    # one of the first step
+    tpr_steps: int = 0
     # topography (&topography itp, &boundaries immb): the DEM is a Gaussian
     # cone made by pdac_amd.topography.cone_dem from the parameters in
     # 'dem' (written as the reference's ASCII DEM file beside the deck)
@@ -289,7 +292,8 @@ class Case:
         lines.append(f" run_name = 'pdac_amd', job_type = '{self.job_type}',")
         lines.append(" restart_mode = 'restart',")
         lines.append(f" time = {d(self.time)}, tstop = {d(tstop)}, dt = {d(self.dt)},")
-        lines.append(f" lpr = {self.lpr}, imr = 0, tpr = {d(1.0e6)}, tdump = {d(self.nsteps * self.dt)},")
+        tpr = self.tpr_steps * self.dt if self.tpr_steps > 0 else 1.0e6
+        lines.append(f" lpr = {self.lpr}, imr = 0, tpr = {d(tpr)}, tdump = {d(self.nsteps * self.dt)},")
         lines.append(f" nfil = 0, max_seconds = {d(1.0e7)}, cstop = {b(self.cstop)}")
         lines.append("/")
         lines.append("&model")

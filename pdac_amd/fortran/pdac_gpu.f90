@@ -19,6 +19,7 @@ MODULE pdac_gpu
   INTEGER(C_INT), PARAMETER :: PDAC_ST_ITER_NOT_CONVERGED = 1, PDAC_ST_EOS_FAILURE = 2, &
        PDAC_ST_BAD_ARGUMENT = 3
   INTEGER(C_INT32_T), PARAMETER :: PDAC_PREC_FP64 = 0, PDAC_PREC_FP32FLUX = 1
+  INTEGER(C_INT), PARAMETER :: PDAC_SNAP_OUTPUT = 0, PDAC_SNAP_RESTART = 1
   INTEGER(C_INT32_T), PARAMETER :: PDAC_ORDER_LEXICOGRAPHIC = 0, PDAC_ORDER_REDBLACK = 1, &
        PDAC_ORDER_STAGE = 2
 
@@ -123,6 +124,27 @@ MODULE pdac_gpu
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx
     END FUNCTION pdac_sgsg
+
+    INTEGER(C_INT) FUNCTION pdac_snapshot_size(ctx, kind, bytes) BIND(C, name='pdac_snapshot_size')
+      IMPORT :: C_INT, C_PTR, C_INT64_T
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: kind
+      INTEGER(C_INT64_T), INTENT(OUT) :: bytes
+    END FUNCTION pdac_snapshot_size
+
+    INTEGER(C_INT) FUNCTION pdac_snapshot_async(ctx, kind) BIND(C, name='pdac_snapshot_async')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: kind
+    END FUNCTION pdac_snapshot_async
+
+    INTEGER(C_INT) FUNCTION pdac_snapshot_fetch(ctx, kind, host, bytes) BIND(C, name='pdac_snapshot_fetch')
+      IMPORT :: C_INT, C_PTR, C_INT64_T
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: kind
+      TYPE(C_PTR), VALUE :: host
+      INTEGER(C_INT64_T), VALUE :: bytes
+    END FUNCTION pdac_snapshot_fetch
 
     INTEGER(C_INT) FUNCTION pdac_begin_step(ctx) BIND(C, name='pdac_begin_step')
       IMPORT :: C_INT, C_PTR

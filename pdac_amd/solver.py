@@ -43,11 +43,15 @@ def bind(lib: ct.CDLL, prefix: str):
         "explicit_iter": [P, ct.c_double, ip, ct.POINTER(ct.c_double)],
     }
     sig["set_ib"] = [P, ip, ip, dp, dp, ip, dp]
+    # device staging of output and restart files (the GPU library only)
+    sig["snapshot_size"] = [P, ct.c_int, ct.POINTER(ct.c_int64)]
+    sig["snapshot_async"] = [P, ct.c_int]
+    sig["snapshot_fetch"] = [P, ct.c_int, ct.c_void_p, ct.c_int64]
     out = {}
     for name, args in sig.items():
         fn = getattr(lib, prefix + name, None)
-        if fn is None and name == "set_ib":
-            continue  # a library without immersed boundaries
+        if fn is None and name in ("set_ib", "snapshot_size", "snapshot_async", "snapshot_fetch"):
+            continue  # a library without immersed boundaries or snapshots
         fn.argtypes = args
         fn.restype = ct.c_int
         out[name] = fn
@@ -124,6 +128,7 @@ class Solver:
         self.time = case.time
         self.nit_last = 0
         self.nit_total = 0
+        self._queued = set()  # snapshot kinds queued and not fetched
         if case.fields:
             self.load_state(case.fields)
 
@@ -189,6 +194,43 @@ class Solver:
 
     def sgsg(self):
         self._check("sgsg", self.fn["sgsg"](self.ctx))
+
+    # -- output and restart files (outp, taperd) ------------------------------
+    def snapshot_async(self, kind):
+        """Pack the arrays of an output (abi.SNAP_OUTPUT) or restart
+        (abi.SNAP_RESTART) file on the device and queue their download;
+        returns at once."""
+        self._check("snapshot_async", self.fn["snapshot_async"](self.ctx, kind))
+        self._queued.add(kind)
+
+    def snapshot_fetch(self, kind):
+        """The arrays queued by snapshot_async (waits for their download):
+        float32 for an output file, float64 for a restart file, one row per
+        array of the file."""
+        nb = ct.c_int64()
+        self._check("snapshot_size", self.fn["snapshot_size"](self.ctx, kind, ct.byref(nb)))
+        dtype = np.float32 if kind == abi.SNAP_OUTPUT else np.float64
+        out = np.empty(nb.value // np.dtype(dtype).itemsize, dtype=dtype)
+        self._check("snapshot_fetch", self.fn["snapshot_fetch"](self.ctx, kind, out.ctypes.data, nb.value))
+        self._queued.discard(kind)
+        return out.reshape(-1, self.case.ntot)
+
+    def write_output(self, path):
+        """The unformatted output file of outp (outp.f:194-384) for the
+        current state and time, from a snapshot queued by
+        ``snapshot_async(abi.SNAP_OUTPUT)`` (queued here if none is)."""
+        from .outputs import write_output
+        if abi.SNAP_OUTPUT not in self._queued:
+            self.snapshot_async(abi.SNAP_OUTPUT)
+        write_output(path, self.time, self.snapshot_fetch(abi.SNAP_OUTPUT))
+
+    def write_restart(self, path, nfil=0):
+        """pdac.res as taperd writes it (io_restart.f:31-70) for the current
+        state and time, from a restart snapshot (queued here if none is)."""
+        from .outputs import write_restart_arrays
+        if abi.SNAP_RESTART not in self._queued:
+            self.snapshot_async(abi.SNAP_RESTART)
+        write_restart_arrays(path, self.case, self.time, nfil, self.snapshot_fetch(abi.SNAP_RESTART))
 
     def begin_step(self):
         self._check("begin_step", self.fn["begin_step"](self.ctx))

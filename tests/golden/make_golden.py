@@ -127,6 +127,15 @@ CASES = {
     "explicit_visc_3d_sgs": lambda: explicit(viscous_box(three_d=True, nsolid=2, west=3, east=4,
                                                          south=4, north=3, top=2, iturb=1, cmut=0.5,
                                                          seed=47, ny=6, nsteps=3)),
+    # output files at the first and the last step (outp) and the restart
+    # file (taperd) kept with the state: 2D cylindrical explicit, 3D
+    # semi-implicit with three classes
+    "explicit_output_2d": lambda: explicit(mixed_box(nsteps=3, itc=1, west=2, east=4, bottom=3,
+                                                     top=2, stratification=True, seed=53,
+                                                     tpr_steps=3)),
+    "output_3d": lambda: mixed_box(three_d=True, nsteps=2, nsolid=3, bottom=3, west=3, east=4,
+                                   south=4, north=3, top=2, stratification=True, seed=59,
+                                   tpr_steps=2),
 }
 
 # Fixtures with a free in/outflow or zero-gradient lid: the reference sets

@@ -49,6 +49,11 @@ def run_reference(case, workdir=None, keep=False, timeout=3600, extras=None):
     nits = [int(l.split("nit:")[1]) for l in log.splitlines() if "nit:" in l]
     if extras is not None:
         extras.update(read_forcing_points(wd))
+        # the output files (outp) and the restart file (taperd) as written
+        for name in sorted(os.listdir(wd)):
+            if name.startswith("output.") or name == "pdac.res":
+                key = "restart" if name == "pdac.res" else name.replace(".", "_")
+                extras[key] = np.frombuffer(open(os.path.join(wd, name), "rb").read(), dtype=np.uint8)
     if not keep and workdir is None:
         shutil.rmtree(wd, ignore_errors=True)
     return st, nits
