@@ -5,12 +5,18 @@
 !   pdac_drive.x --sizes          print C_SIZEOF of pdac_dims and pdac_params
 !                                 (no device call)
 !   pdac_drive.x IN OUT           read a domain from IN, step it, write OUT
+!   pdac_drive.x IN OUT DIR       the same, and in DIR the output file of the
+!                                 first step's boundary call (output.0000), the
+!                                 one of the last step (output.0001) and the
+!                                 restart file (pdac.res), written here with
+!                                 the reference's unformatted records from
+!                                 device snapshots (outp, taperd)
 !
 ! IN is a little-endian stream file: the dims (8 int32); the params
 ! component by component (12 doubles, dk rl phis cps kap plim of
 ! PDAC_MAX_SOLID doubles each, atm_ygc of PDAC_MAX_GAS doubles, gas_type of
 ! PDAC_MAX_GAS int32, 16 int32, mus of PDAC_MAX_SOLID doubles, cmut); nsteps,
-! rungekut, mode (int32); dt (double);
+! rungekut, mode (int32); dt, the start time (double);
 ! immb (int32); dx(nx), dy(ny), dz(nz), p_atm(nz), t_atm(nz) (doubles);
 ! flag(ntot) (int32); when immb = 1 the immersed-boundary arrays of
 ! pdac_set_ib (nfp(3), fpi(7*sum nfp), fpd(3*sum nfp), coords(2nx+2ny+2nz),
@@ -31,11 +37,12 @@ PROGRAM pdac_drive
   TYPE(pdac_dims) :: dims
   TYPE(pdac_params) :: prm
   TYPE(C_PTR) :: ctx
-  CHARACTER(LEN=1024) :: arg_in, arg_out
+  CHARACTER(LEN=1024) :: arg_in, arg_out, arg_dir
   INTEGER(C_INT32_T) :: d8(8), pint(16), nsteps, rungekut, mode, immb, nput, nget, fid
   INTEGER(C_INT32_T) :: nfp(3), nit, conv, fail, nit_last
   INTEGER(C_INT64_T) :: nit_total, cnt, ntot
-  REAL(C_DOUBLE) :: pdbl(12), dt, dts, cfl
+  REAL(C_DOUBLE) :: pdbl(12), dt, dts, cfl, time
+  INTEGER(C_INT32_T) :: nfil
   REAL(C_DOUBLE), ALLOCATABLE :: dx(:), dy(:), dz(:), p_atm(:), t_atm(:), fpd(:), coords(:), &
        ivf(:), a(:)
   INTEGER(C_INT32_T), ALLOCATABLE :: flag(:), fpi(:), bmask(:), ids(:)
@@ -46,11 +53,13 @@ PROGRAM pdac_drive
   IF (COMMAND_ARGUMENT_COUNT() >= 1) CALL GET_COMMAND_ARGUMENT(1, arg_in)
   IF (COMMAND_ARGUMENT_COUNT() == 1 .AND. TRIM(arg_in) == '--sizes') THEN
     WRITE(*, '(A,I0,A,I0)') 'pdac_dims ', C_SIZEOF(dims), ' pdac_params ', C_SIZEOF(prm)
-  ELSE IF (COMMAND_ARGUMENT_COUNT() == 2) THEN
+  ELSE IF (COMMAND_ARGUMENT_COUNT() == 2 .OR. COMMAND_ARGUMENT_COUNT() == 3) THEN
     CALL GET_COMMAND_ARGUMENT(2, arg_out)
+    arg_dir = ''
+    IF (COMMAND_ARGUMENT_COUNT() == 3) CALL GET_COMMAND_ARGUMENT(3, arg_dir)
     CALL run()
   ELSE
-    WRITE(*, '(A)') 'usage: pdac_drive.x --sizes | pdac_drive.x IN OUT'
+    WRITE(*, '(A)') 'usage: pdac_drive.x --sizes | pdac_drive.x IN OUT [DIR]'
     ERROR STOP 2
   END IF
 
@@ -75,7 +84,7 @@ CONTAINS
     prm%gas_viscosity = pint(13); prm%part_viscosity = pint(14); prm%repulsive_model = pint(15)
     prm%iturb = pint(16)
     READ(u) prm%mus, prm%cmut
-    READ(u) nsteps, rungekut, mode, dt, immb
+    READ(u) nsteps, rungekut, mode, dt, time, immb
     ntot = INT(dims%nx, C_INT64_T) * dims%ny * dims%nz
     ALLOCATE(dx(dims%nx), dy(dims%ny), dz(dims%nz), p_atm(dims%nz), t_atm(dims%nz), flag(ntot))
     READ(u) dx, dy, dz, p_atm, t_atm, flag
@@ -113,6 +122,7 @@ CONTAINS
 
     nit_last = 0
     nit_total = 0
+    nfil = 0
     IF (mode == 1) THEN
       rc = pdac_step(ctx, dt, rungekut, nsteps, nit_last, nit_total)
       CALL check('pdac_step')
@@ -120,6 +130,8 @@ CONTAINS
       DO istep = 1, nsteps
         rc = pdac_boundary(ctx)
         CALL check('boundary')
+        ! the initial conditions' output file (prog.f:160-165)
+        IF (istep == 1 .AND. LEN_TRIM(arg_dir) > 0) CALL write_output()
         IF (prm%time_integration == 1) THEN
           rc = pdac_initialize_fieldn(ctx, dt)
           CALL check('initialize_fieldn')
@@ -155,7 +167,12 @@ CONTAINS
             CALL check('ftem')
           END DO
         END IF
+        time = time + dt
       END DO
+      IF (LEN_TRIM(arg_dir) > 0) THEN
+        CALL write_output()
+        CALL write_restart()
+      END IF
     END IF
 
     OPEN(NEWUNIT=u, FILE=TRIM(arg_out), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', &
@@ -172,6 +189,56 @@ CONTAINS
     CLOSE(u)
     rc = pdac_destroy(ctx)
   END SUBROUTINE run
+
+  ! outp (outp.f:194-384), unformatted: the time as a 4-byte real, then the
+  ! arrays of a device snapshot one record each
+  SUBROUTINE write_output()
+    REAL(C_FLOAT), ALLOCATABLE, TARGET :: fb(:, :)
+    INTEGER(C_INT64_T) :: nb
+    CHARACTER(LEN=4) :: num
+    INTEGER :: v, j
+    rc = pdac_snapshot_async(ctx, PDAC_SNAP_OUTPUT)
+    CALL check('pdac_snapshot_async')
+    rc = pdac_snapshot_size(ctx, PDAC_SNAP_OUTPUT, nb)
+    CALL check('pdac_snapshot_size')
+    ALLOCATE(fb(ntot, nb / (4 * ntot)))
+    rc = pdac_snapshot_fetch(ctx, PDAC_SNAP_OUTPUT, C_LOC(fb), nb)
+    CALL check('pdac_snapshot_fetch')
+    WRITE(num, '(I4.4)') nfil
+    OPEN(NEWUNIT=v, FILE=TRIM(arg_dir)//'/output.'//num, FORM='UNFORMATTED', STATUS='REPLACE', &
+         ACTION='WRITE')
+    WRITE(v) REAL(time, 4)
+    DO j = 1, SIZE(fb, 2)
+      WRITE(v) fb(:, j)
+    END DO
+    CLOSE(v)
+    DEALLOCATE(fb)
+    nfil = nfil + 1
+  END SUBROUTINE write_output
+
+  ! taperd (io_restart.f:31-70): the header records, then the arrays of a
+  ! device snapshot one record each
+  SUBROUTINE write_restart()
+    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: db(:, :)
+    INTEGER(C_INT64_T) :: nb
+    INTEGER :: v, j
+    rc = pdac_snapshot_async(ctx, PDAC_SNAP_RESTART)
+    CALL check('pdac_snapshot_async')
+    rc = pdac_snapshot_size(ctx, PDAC_SNAP_RESTART, nb)
+    CALL check('pdac_snapshot_size')
+    ALLOCATE(db(ntot, nb / (8 * ntot)))
+    rc = pdac_snapshot_fetch(ctx, PDAC_SNAP_RESTART, C_LOC(db), nb)
+    CALL check('pdac_snapshot_fetch')
+    OPEN(NEWUNIT=v, FILE=TRIM(arg_dir)//'/pdac.res', FORM='UNFORMATTED', STATUS='REPLACE', &
+         ACTION='WRITE')
+    WRITE(v) time, dims%nx, dims%ny, dims%nz, dims%nsolid, dims%ngas, nfil
+    WRITE(v) prm%gas_type(1:dims%ngas)
+    DO j = 1, SIZE(db, 2)
+      WRITE(v) db(:, j)
+    END DO
+    CLOSE(v)
+    DEALLOCATE(db)
+  END SUBROUTINE write_restart
 
   ! the reference's CALL error(...) on a failing call: message and stop
   SUBROUTINE check(where)

@@ -103,7 +103,7 @@ def write_domain(path, case, nsteps, mode, dt=None, rungekut=None, precision=abi
                       p.part_viscosity, p.repulsive_model, p.iturb]))
     parts.append(f64(list(p.mus) + [p.cmut]))
     parts.append(i32([nsteps, case.rungekut if rungekut is None else rungekut, mode]))
-    parts.append(f64([case.dt if dt is None else dt]))
+    parts.append(f64([case.dt if dt is None else dt, case.time]))
     immb = int(getattr(case, "immb", 0) or 0)
     parts.append(i32([immb]))
     parts += [f64(dx), f64(dy if case.three_d else np.zeros(1)), f64(dz), f64(p_atm), f64(t_atm),
@@ -167,3 +167,35 @@ def test_fortran_driver_matches_python_host(tmp_path, name, mode, nsteps):
         py = np.asarray(g.get(nm)).reshape(-1)
         assert np.array_equal(fort[nm], py), nm
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["explicit_output_2d", "output_3d"])
+def test_fortran_driver_writes_the_reference_files(tmp_path, name):
+    """The driver writes output.0000 (after the first step's boundary call),
+    output.0001 and pdac.res with Fortran unformatted records from device
+    snapshots, byte for byte the files the Python host writes for the same
+    steps; their layout is the reference's (the Python writers match the
+    reference's own files, tests/test_outputs.py)."""
+    from pdac_amd.gpu import gpu_solver
+    from tests.golden.make_golden import load
+    case, _, _ = load(name)
+    write_domain(tmp_path / "in.bin", case, case.nsteps, 0)
+    fdir = tmp_path / "fortran"
+    fdir.mkdir()
+    print(f"[fortran driver files] {name}", flush=True)
+    res = subprocess.run([driver(), str(tmp_path / "in.bin"), str(tmp_path / "out.bin"), str(fdir)],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    g = gpu_solver(case, device=0)
+    g.boundary()
+    g.write_output(str(tmp_path / "output.0000"))
+    g.close()
+    g = gpu_solver(case, device=0)
+    for _ in range(case.nsteps):
+        g.step_by_parts()
+    g.write_output(str(tmp_path / "output.0001"))
+    g.write_restart(str(tmp_path / "pdac.res"), nfil=2)
+    g.close()
+    for f in ("output.0000", "output.0001", "pdac.res"):
+        assert (fdir / f).read_bytes() == (tmp_path / f).read_bytes(), f
