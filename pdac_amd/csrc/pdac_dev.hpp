@@ -89,6 +89,9 @@ struct Fields {
   // cell (smag); null without it
   double *mugt, *kapgt;
   const double *smag;
+  // gas-particle drag coefficient of each class (ntot, S): k_drag writes it
+  // once per cell for the axis threads of k_tilde
+  double *kpgv;
 };
 
 // face masks of a cell (b_e_ .. b_b_, immb.f:56-58): bits of Fields::ibm

@@ -774,6 +774,32 @@ __device__ __forceinline__ void inter_axis(const double *kpgv, double dt, const 
 // the threads, each with a third of the axis work and fewer registers live
 // (three waves per SIMD), keep more of them in flight: 5.96 -> 4.53 ms on the
 // 256x256x192 cone.
+// the gas-particle drag coefficient of each class at a computed cell
+// (kdrags or sdrag, drag.f:11-157, from the cell-centred slip velocities),
+// once per cell for the axis threads of k_tilde
+template <int D, int NS>
+__global__ void __launch_bounds__(kBlock) k_drag(Grid g, Consts K, Fields F) {
+  const long c = cell_id();
+  if (c >= g.n || !computed(g.flag[c])) return;
+  const long n = g.n;
+  int i, j, k;
+  decode(g, c, i, j, k);
+  Nbr<D> q;
+  first_nbrs<D>(g, i, j, k, q);
+  const double *ug = F.f[PDAC_UG], *vg = F.f[PDAC_VG], *wg = F.f[PDAC_WG];
+  const double *us = F.f[PDAC_US], *vs = F.f[PDAC_VS], *ws = F.f[PDAC_WS];
+  const double *rgp = F.f[PDAC_RGP], *rlk = F.f[PDAC_RLK], *ep = F.f[PDAC_EP], *mug = F.f[PDAC_MUG];
+#pragma unroll
+  for (int is = 0; is < NS; ++is) {
+    const long os = (long)is * n;
+    const double dugs = ((ug[c] - us[os + c]) + (ug[q.imjk] - us[os + q.imjk])) * 0.5;
+    const double dwgs = ((wg[c] - ws[os + c]) + (wg[q.ijkm] - ws[os + q.ijkm])) * 0.5;
+    const double dvgs = (D == 3) ? ((vg[c] - vs[os + c]) + (vg[q.ijmk] - vs[os + q.ijmk])) * 0.5 : 0.0;
+    F.kpgv[os + c] = K.drag_model == 1 ? kdrags(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c])
+                                       : sdrag(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c]);
+  }
+}
+
 // (VISC: with the viscous stress divergence of k_stress)
 template <int D, int NS, typename FT, bool VISC>
 __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F, Ib ib, double dt, int old_zero) {
@@ -793,19 +819,15 @@ __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F,
   const double *rgpn = F.f[PDAC_RGPN], *rlkn = F.f[PDAC_RLKN], *pn = F.f[PDAC_PN], *epn = F.f[PDAC_EPN];
   const double *ug = F.f[PDAC_UG], *vg = F.f[PDAC_VG], *wg = F.f[PDAC_WG];
   const double *us = F.f[PDAC_US], *vs = F.f[PDAC_VS], *ws = F.f[PDAC_WS];
-  const double *rgp = F.f[PDAC_RGP], *rlk = F.f[PDAC_RLK], *ep = F.f[PDAC_EP], *mug = F.f[PDAC_MUG];
+  const double *rgp = F.f[PDAC_RGP], *rlk = F.f[PDAC_RLK], *ep = F.f[PDAC_EP];
   const double alpha = K.alpha, ag = K.alphagrav;
-  // drag coefficients and particle-particle factors (drag.f:60-157, 265-295)
+  // drag coefficients (k_drag) and particle-particle factors (drag.f:265-295)
   double kpgv[NS], rlkc[NS], con[NPAIR > 0 ? NPAIR : 1];
 #pragma unroll
   for (int is = 0; is < NS; ++is) {
-    long os = (long)is * n;
+    const long os = (long)is * n;
     rlkc[is] = rlk[os + c];
-    double dugs = ((ug[c] - us[os + c]) + (ug[q.imjk] - us[os + q.imjk])) * 0.5;
-    double dwgs = ((wg[c] - ws[os + c]) + (wg[q.ijkm] - ws[os + q.ijkm])) * 0.5;
-    double dvgs = (D == 3) ? ((vg[c] - vs[os + c]) + (vg[q.ijmk] - vs[os + q.ijmk])) * 0.5 : 0.0;
-    if (K.drag_model == 1) kpgv[is] = kdrags(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c]);
-    else kpgv[is] = sdrag(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c]);
+    kpgv[is] = F.kpgv[os + c];
   }
   {
     int pc = 0;
