@@ -271,16 +271,22 @@ def topo_setup(case, dem: Dem) -> TopoSetup:
 def export_layer(case, fl):
     """export_topography (topo.f:1114-1162): fluid cells within three rows
     above the last no-slip cell met (in the loop over j, i, k, the row
-    counter carrying over from column to column) become bl_cell."""
+    counter carrying over from column to column) become bl_cell.
+
+    The serial loop, vectorised: along a column the counter is the highest
+    no-slip row met so far (rows only grow within a column); before the
+    column's first no-slip cell it is the last one of the nearest earlier
+    column that has one (the initial bl_row0 if none)."""
     nx, ny, nz = case.nx, case.ny, case.nz
-    flv = fl.reshape(nz, ny, nx)
-    q = case.bl_row0
-    for j in range(ny):
-        for i in range(nx):
-            col = flv[:, j, i]
-            for k in range(nz):
-                if col[k] == C.NOSLIP_WALL:
-                    q = k + 1
-                elif col[k] == C.FLUID and (k + 1) <= q + 3:
-                    col[k] = C.BL_CELL
+    flv = fl.reshape(nz, ny * nx)                    # columns in the loop's j, i order
+    rows = np.arange(1, nz + 1, dtype=np.int64)[:, None]
+    ns = flv == C.NOSLIP_WALL
+    q_in = np.maximum.accumulate(np.where(ns, rows, 0), axis=0)   # 0 until the first no-slip
+    last = q_in[-1]                                  # last no-slip row of each column, 0 if none
+    idx = np.where(last > 0, np.arange(last.size), -1)
+    np.maximum.accumulate(idx, out=idx)              # nearest column at or before with one
+    carry = np.where(idx >= 0, last[np.maximum(idx, 0)], case.bl_row0)
+    carry = np.concatenate([[case.bl_row0], carry[:-1]])   # what the previous columns leave
+    q = np.where(q_in > 0, q_in, carry[None, :])
+    flv[(flv == C.FLUID) & (rows <= q + 3)] = C.BL_CELL
     return fl

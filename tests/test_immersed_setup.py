@@ -86,3 +86,34 @@ def test_bilinear_interpolation_closed_forms():
 
 def test_nint_rounds_halves_away_from_zero():
     assert fortran_nint([0.5, 1.5, 2.5, -0.5, -2.5, 2.4999]).tolist() == [1, 2, 3, -1, -3, 2]
+
+
+def test_export_layer_equals_the_serial_loop():
+    """export_layer is export_topography's serial loop (topo.f:1114-1162)
+    vectorised: on random small flag fields, every column order and carry of
+    the row counter, it gives what the loop gives cell by cell."""
+    from types import SimpleNamespace
+
+    import pdac_amd.constants as C
+    from pdac_amd.topography import export_layer
+
+    def serial(case, fl):
+        flv = fl.reshape(case.nz, case.ny, case.nx)
+        q = case.bl_row0
+        for j in range(case.ny):
+            for i in range(case.nx):
+                col = flv[:, j, i]
+                for k in range(case.nz):
+                    if col[k] == C.NOSLIP_WALL:
+                        q = k + 1
+                    elif col[k] == C.FLUID and (k + 1) <= q + 3:
+                        col[k] = C.BL_CELL
+        return fl
+
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        nx, ny, nz = (int(v) for v in rng.integers(1, 7, 3))
+        case = SimpleNamespace(nx=nx, ny=ny, nz=nz, bl_row0=int(rng.integers(0, 4)))
+        fl = rng.choice([C.FLUID, C.NOSLIP_WALL, C.SLIP_WALL, C.FREE_IO], size=nx * ny * nz,
+                        p=[0.7, 0.15, 0.1, 0.05]).astype(np.int32)
+        assert np.array_equal(export_layer(case, fl.copy()), serial(case, fl.copy()))
