@@ -92,7 +92,18 @@ struct Fields {
   // gas-particle drag coefficient of each class (ntot, S): k_drag writes it
   // once per cell for the axis threads of k_tilde
   double *kpgv;
+  // particle classes present near each tile of 4x4x4 cells: bit s of ptile
+  // is set when a cell of the tile or of a face/edge/corner-adjacent tile
+  // holds class s (rlk != 0); ptile_own before that dilation.  tnx, tny:
+  // tiles along x and y.  Null: no tile test (every class is computed).
+  uint32_t *ptile, *ptile_own;
+  int tnx, tny, tnz;
 };
+
+// tile of 4x4x4 cells holding cell (i, j, k), 1-based
+__host__ __device__ __forceinline__ long phase_tile(const Fields &F, int i, int j, int k) {
+  return (long)((i - 1) >> 2) + (long)F.tnx * ((long)((j - 1) >> 2) + (long)F.tny * (long)((k - 1) >> 2));
+}
 
 // face masks of a cell (b_e_ .. b_b_, immb.f:56-58): bits of Fields::ibm
 enum : uint8_t { IBM_E = 1, IBM_W = 2, IBM_N = 4, IBM_S = 8, IBM_T = 16, IBM_B = 32, IBM_VF = 64 };

@@ -655,6 +655,76 @@ __device__ __forceinline__ void momflux_cell(const Grid &g, const Consts &K, con
   }
 }
 
+// Classes present near each tile (Fields::ptile), from the particle
+// densities of the current state.  A momentum or enthalpy flux of class s at
+// a cell reads that class's densities within three cells of it, so a tile
+// without the class's bit has every such flux a product with a zero density:
+// zero.  Gated like the kernels after the iteration (iter_ended).
+__global__ void __launch_bounds__(kBlock) k_ptile_clear(Fields F, long ntiles, const int32_t *ctr, int gate,
+                                                        int maxout) {
+  if (!iter_ended(ctr, gate, maxout)) return;
+  const long t = cell_id();
+  if (t < ntiles) F.ptile_own[t] = 0u;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kBlock) k_ptile_mark(Grid g, Fields F, const int32_t *ctr, int gate, int maxout) {
+  if (!iter_ended(ctr, gate, maxout)) return;
+  const long c = cell_id();
+  if (c >= g.n) return;
+  uint32_t bits = 0;
+#pragma unroll
+  for (int is = 0; is < NS; ++is)
+    if (F.f[PDAC_RLK][(long)is * g.n + c] != 0.0) bits |= 1u << is;
+  if (!bits) return;
+  int i, j, k;
+  decode(g, c, i, j, k);
+  __hip_atomic_fetch_or(F.ptile_own + phase_tile(F, i, j, k), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(kBlock) k_ptile_dilate(Fields F, const int32_t *ctr, int gate, int maxout) {
+  if (!iter_ended(ctr, gate, maxout)) return;
+  const long t = cell_id();
+  const long ntiles = (long)F.tnx * F.tny * F.tnz;
+  if (t >= ntiles) return;
+  const int tx = (int)(t % F.tnx), ty = (int)((t / F.tnx) % F.tny), tz = (int)(t / ((long)F.tnx * F.tny));
+  uint32_t v = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = tx + dx, y = ty + dy, z = tz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= F.tnx || y >= F.tny || z >= F.tnz) continue;
+        v |= F.ptile_own[x + (long)F.tnx * (y + (long)F.tny * z)];
+      }
+  F.ptile[t] = v;
+}
+
+// the faces k_momflux writes for one phase of a computed cell, set to zero
+// (a class absent near the cell: its fluxes are products with zero densities)
+template <int D, typename FT>
+__device__ __forceinline__ void momflux_zero(const Grid &g, const Fields &F, long c, int ph, int i, int j, int k) {
+  const long o = (long)ph * g.n;
+  const int32_t *fl = g.flag;
+  const long w = rawn(g, i, j, k, -1, 0, 0), b = rawn(g, i, j, k, 0, 0, -1);
+  const long s = D == 3 ? rawn(g, i, j, k, 0, -1, 0) : c;
+  const bool hw = !computed(fl[w]), hb = !computed(fl[b]), hs = D == 3 && !computed(fl[s]);
+  FT *m[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) m[a][bb] = (FT *)F.mf[a][bb];
+  const FT z = FT(0);
+  m[0][0][o + c] = z; m[0][2][o + c] = z; m[2][0][o + c] = z; m[2][2][o + c] = z;
+  if (hw) { m[0][0][o + w] = z; m[2][0][o + w] = z; }
+  if (hb) { m[0][2][o + b] = z; m[2][2][o + b] = z; }
+  if (D == 3) {
+    m[0][1][o + c] = z; m[1][0][o + c] = z; m[1][1][o + c] = z; m[1][2][o + c] = z; m[2][1][o + c] = z;
+    if (hw) m[1][0][o + w] = z;
+    if (hb) m[1][2][o + b] = z;
+    if (hs) { m[0][1][o + s] = z; m[1][1][o + s] = z; m[2][1][o + s] = z; }
+  }
+}
+
 // FP32 fluxes: at least three waves per SIMD (168 VGPRs, a few spilled): the
 // kernel is bound by the latency of its gathers (4.00 -> 3.72 ms on the
 // 256x256x192 cone against two waves at 182 VGPRs).  FP64 fluxes need 238
@@ -671,6 +741,14 @@ __global__ void __launch_bounds__(kBlock, sizeof(FT) == 4 ? 3 : 2) k_momflux(Gri
   else {
     long o = (long)(ph - 1) * g.n;
     uu = F.f[PDAC_US] + o; vv = F.f[PDAC_VS] + o; ww = F.f[PDAC_WS] + o; dd = F.f[PDAC_RLK] + o;
+  }
+  if (ph > 0 && F.ptile) {
+    int i, j, k;
+    decode(g, c, i, j, k);
+    if (!((F.ptile[phase_tile(F, i, j, k)] >> (ph - 1)) & 1u)) {
+      momflux_zero<D, FT>(g, F, c, ph, i, j, k);
+      return;
+    }
   }
   momflux_cell<D, FT>(g, K, F, c, fc, ph, uu, vv, ww, dd);
 }
