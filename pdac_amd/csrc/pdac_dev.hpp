@@ -98,11 +98,27 @@ struct Fields {
   // tiles along x and y.  Null: no tile test (every class is computed).
   uint32_t *ptile, *ptile_own;
   int tnx, tny, tnz;
+  // the tiles listed for the particle-class flux kernels of a 3D mesh (tlist,
+  // *tcount entries): those with a class present near them now or at the
+  // consuming kernel's previous call.  Bits 16.. of ptile mark the classes
+  // present at that call only, whose fluxes the consumer sets to zero.
+  // ptile_mprev, ptile_hprev: the class bits of the momentum and enthalpy
+  // flux kernels' previous calls (0xffff: unknown).
+  int32_t *tlist, *tcount;
+  uint32_t *ptile_mprev, *ptile_hprev;
 };
 
 // tile of 4x4x4 cells holding cell (i, j, k), 1-based
 __host__ __device__ __forceinline__ long phase_tile(const Fields &F, int i, int j, int k) {
   return (long)((i - 1) >> 2) + (long)F.tnx * ((long)((j - 1) >> 2) + (long)F.tny * (long)((k - 1) >> 2));
+}
+
+// cell (i, j, k), 1-based, of lane 0..63 of tile t (x fastest, then y, z)
+__device__ __forceinline__ void tile_cell(const Fields &F, long t, int lane, int &i, int &j, int &k) {
+  const int tx = (int)(t % F.tnx), ty = (int)((t / F.tnx) % F.tny), tz = (int)(t / ((long)F.tnx * F.tny));
+  i = 4 * tx + (lane & 3) + 1;
+  j = 4 * ty + ((lane >> 2) & 3) + 1;
+  k = 4 * tz + (lane >> 4) + 1;
 }
 
 // face masks of a cell (b_e_ .. b_b_, immb.f:56-58): bits of Fields::ibm

@@ -664,7 +664,18 @@ __global__ void __launch_bounds__(kBlock) k_ptile_clear(Fields F, long ntiles, c
                                                         int maxout) {
   if (!iter_ended(ctr, gate, maxout)) return;
   const long t = cell_id();
+  if (t == 0) *F.tcount = 0;
   if (t < ntiles) F.ptile_own[t] = 0u;
+}
+
+// both consumers' previous class bits made unknown (every class of every
+// tile is computed or zeroed at their next call): launched where other
+// kernels write the particle flux arrays (the explicit scheme)
+__global__ void __launch_bounds__(kBlock) k_ptile_unknown(Fields F, long ntiles) {
+  const long t = cell_id();
+  if (t >= ntiles) return;
+  F.ptile_mprev[t] = 0xffffu;
+  F.ptile_hprev[t] = 0xffffu;
 }
 
 template <int NS>
@@ -682,7 +693,10 @@ __global__ void __launch_bounds__(kBlock) k_ptile_mark(Grid g, Fields F, const i
   __hip_atomic_fetch_or(F.ptile_own + phase_tile(F, i, j, k), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ void __launch_bounds__(kBlock) k_ptile_dilate(Fields F, const int32_t *ctr, int gate, int maxout) {
+// prev: the consumer's class bits of its previous call, replaced by this
+// call's; tiles with a class now or then go to the list (one atomic per wave)
+__global__ void __launch_bounds__(kBlock) k_ptile_dilate(Fields F, uint32_t *prev, const int32_t *ctr, int gate,
+                                                         int maxout) {
   if (!iter_ended(ctr, gate, maxout)) return;
   const long t = cell_id();
   const long ntiles = (long)F.tnx * F.tny * F.tnz;
@@ -696,7 +710,17 @@ __global__ void __launch_bounds__(kBlock) k_ptile_dilate(Fields F, const int32_t
         if (x < 0 || y < 0 || z < 0 || x >= F.tnx || y >= F.tny || z >= F.tnz) continue;
         v |= F.ptile_own[x + (long)F.tnx * (y + (long)F.tny * z)];
       }
-  F.ptile[t] = v;
+  const uint32_t gone = prev[t] & ~v & 0xffffu;
+  F.ptile[t] = v | (gone << 16);
+  prev[t] = v;
+  const bool want = (v | gone) != 0u;
+  const uint64_t m = __ballot(want);
+  if (!m) return;
+  const int lane = threadIdx.x & 63, leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(F.tcount, __popcll(m));
+  base = __shfl(base, leader);
+  if (want) F.tlist[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)t;
 }
 
 // the faces k_momflux writes for one phase of a computed cell, set to zero
@@ -751,6 +775,34 @@ __global__ void __launch_bounds__(kBlock, sizeof(FT) == 4 ? 3 : 2) k_momflux(Gri
     }
   }
   momflux_cell<D, FT>(g, K, F, c, fc, ph, uu, vv, ww, dd);
+}
+
+// The particle classes' momentum fluxes of a 3D mesh over the listed tiles
+// (Fields::tlist), a wave per tile and class (blockIdx.y + 1), a lane per
+// cell of the tile: a class present near the tile is computed, one present at
+// the previous call only is set to zero (its old values would stand); the
+// other classes' fluxes hold the zeros of such a call or of creation.
+template <int NS, typename FT>
+__global__ void __launch_bounds__(kBlock, sizeof(FT) == 4 ? 3 : 2) k_momflux_tiles(Grid g, Consts K, Fields F) {
+  const int ph = blockIdx.y + 1, lane = threadIdx.x & 63;
+  const long o = (long)(ph - 1) * g.n;
+  const double *uu = F.f[PDAC_US] + o, *vv = F.f[PDAC_VS] + o, *ww = F.f[PDAC_WS] + o, *dd = F.f[PDAC_RLK] + o;
+  const int ntl = *F.tcount;
+  const long stride = (long)gridDim.x * (kBlock / 64);
+  for (long w = (long)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < ntl; w += stride) {
+    const long t = F.tlist[w];
+    const uint32_t bits = F.ptile[t];
+    const bool on = (bits >> (ph - 1)) & 1u, off = (bits >> (ph + 15)) & 1u;
+    if (!on && !off) continue;
+    int i, j, k;
+    tile_cell(F, t, lane, i, j, k);
+    if (i > g.nx || j > g.ny || k > g.nz) continue;
+    const long c = cell_index(g, i, j, k);
+    const int fc = g.flag[c];
+    if (!computed(fc)) continue;
+    if (on) momflux_cell<3, FT>(g, K, F, c, fc, ph, uu, vv, ww, dd);
+    else momflux_zero<3, FT>(g, F, c, ph, i, j, k);
+  }
 }
 
 // inter, drag.f:173-246: packed symmetric interphase matrix with zero row sums
