@@ -138,6 +138,42 @@ def test_momentum_fluxes(name):
             assert rel(got, want) <= 1e-14, (comp, face, rel(got, want))
 
 
+@pytest.mark.parametrize("name", ["mixed_3d", "cone_3d_ib"])
+def test_fluxes_after_classes_leave(name):
+    """3D meshes compute the particle classes' momentum and enthalpy fluxes
+    only on the 4x4x4 tiles near cells holding the class, and zero those of
+    the tiles the class has left since the previous call.  Class 2 removed
+    everywhere and class 1 kept in the first tile only, between two calls:
+    the fluxes and the htilde state equal the oracle's."""
+    import ctypes as ct
+    case, _, _ = load(name)
+    g, c = gpu(case), cpu_solver(case, iter_order=abi.ORDER_REDBLACK)
+    sync(g, c)
+    for s in (g, c):
+        s.boundary(); s.fieldn(); s.begin_step(); s.tilde(case.dt); s.htilde(case.dt)
+    rlk = c.get("rlk").copy()
+    nx, ny, nz = case.nx, case.ny, case.nz
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    first_tile = ((i < 4) & (j < 4) & (k < 4)).transpose(2, 1, 0).reshape(-1)
+    rlk[1] = np.where(first_tile, rlk[1], 0.0)
+    rlk[2] = 0.0
+    assert np.count_nonzero(rlk[0]) and np.count_nonzero(rlk[1])
+    for s in (g, c):
+        s.put("rlk", rlk)
+        s.begin_step(); s.tilde(case.dt)
+    for comp in range(3):
+        for face in range(3):
+            want = np.empty(case.ntot * (case.nsolid + 1))
+            assert c.lib.pdac_cpu_get_momflux(c.ctx, ct.c_int(comp), ct.c_int(face),
+                                              want.ctypes.data_as(ct.POINTER(ct.c_double))) == 0
+            assert rel(g.momflux(comp, face).reshape(-1), want) <= 1e-14, (comp, face)
+    g.htilde(case.dt)
+    c.htilde(case.dt)
+    compare(g, c, 1e-14, f"{name} htilde after classes left")
+    g.close()
+    c.close()
+
+
 @pytest.mark.parametrize("mode", ["persistent", "launches", "graph", "launches:2:late", "launches:5:flush"])
 @pytest.mark.parametrize("name", ALL)
 def test_steps_match_checker(name, mode):
