@@ -188,6 +188,23 @@ int pdac_snapshot_fetch(pdac_ctx *ctx, int kind, void *host, int64_t bytes);
 int pdac_set_ib(pdac_ctx *ctx, const int32_t *nfp, const int32_t *fpi, const double *fpd,
                 const double *coords, const int32_t *bmask, const double *ivf);
 
+/* Immersed-boundary set-up on the host (no device call): the forcing points
+ * of one velocity component (axis 1..3) of an nx x ny x nz mesh, as
+ * set_forcing makes them (immb.f:73-516): first one per cell of ext_cells
+ * (0-based cells, ascending: the component lies above the ground next to a
+ * filled cell; ext_forcing3d, immb.f:1071-1261), then those of the interior
+ * columns in (i, j, k) order (forcing3d, immb.f:913-1069).  ord2d and topo2d
+ * hold one value per column at (i-1)*ny + (j-1): the lowest level above the
+ * ground and the ground height on the component's grid (interpolate_dem,
+ * topo.f:970-1029); cx, cy, cz the grid's coordinates (nx, ny, nz values).
+ * Output: *count points, cap at most, as rows of pts (i, j, k, int, di, dj,
+ * dk: TYPE forcing_point, immb.f:33-45) and nsl (the no-slip point x, y, z).
+ * Columns are split over host threads; the result does not depend on them.
+ * PDAC_ST_BAD_ARGUMENT with *count set when more than cap points. */
+int pdac_ib_forcing(int32_t nx, int32_t ny, int32_t nz, int32_t axis, const int32_t *ord2d, const double *cx,
+                    const double *cy, const double *cz, const double *topo2d, const int64_t *ext_cells,
+                    int64_t n_ext, int64_t cap, int32_t *pts, double *nsl, int64_t *count);
+
 /* Host <-> device copies of one field; n must equal ntot*ncomp(field). */
 int pdac_put(pdac_ctx *ctx, int field, const double *host, int64_t n);
 int pdac_get(pdac_ctx *ctx, int field, double *host, int64_t n);

@@ -47,10 +47,10 @@ def fill_state(case: Case, p, tg, ts, eps, ygc, ug=None, vg=None, wg=None,
         mg = mg + xgc[g] * w[g]
     rog = p / (C.RGAS * tg) * mg
     rgp = ep * rog
-    cp = C.hcapg(tg, case.icpc, case.tref)
+    cp = C.hcapg(tg, case.icpc, case.tref, species=[gt - 1 for gt in case.gas_type])
     cg = np.zeros(n)
-    for g, gt in enumerate(case.gas_type):
-        cg = cg + cp[gt - 1] * ygc[g]
+    for g in range(len(case.gas_type)):
+        cg = cg + cp[g] * ygc[g]
     sieg = (tg - C.TZERO) * cg + C.HZEROG
     cps = np.asarray(case.specific_heat, dtype=np.float64)[:ns].reshape(-1, 1)
     ts = np.asarray(ts, dtype=np.float64).reshape(ns, n)

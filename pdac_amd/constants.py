@@ -40,20 +40,21 @@ _CP_COEF = [
 ]
 
 
-def hcapg(tg, icpc=2, tref=298.0):
+def hcapg(tg, icpc=2, tref=298.0, species=None):
     """Specific heats (J/(kg K)) of the 7 gas species at temperature ``tg``.
 
     Restates hcapgs.f:33-85 with the same left-to-right evaluation order.
-    ``tg`` may be a numpy array; the result has a leading axis of 7."""
+    ``tg`` may be a numpy array; the result has a leading axis of 7, or of
+    len(species) for the 0-based species listed."""
     t1 = np.asarray(tg, dtype=np.float64) if icpc == 2 else np.full_like(
         np.asarray(tg, dtype=np.float64), tref)
     t2 = t1 * t1
     t3 = t1 * t2
-    out = []
-    for c0, c1, c2, c3 in _CP_COEF:
-        out.append(((c0 + c1 * t1) + c2 * t2) + c3 * t3)
-    cp = np.stack(out) / GMW.reshape((7,) + (1,) * t1.ndim)
-    if icpc == 0:
-        cp[4] = 1810.0
-        cp[5] = 1004.0
+    rows = list(range(7)) if species is None else [int(s) for s in species]
+    cp = np.empty((len(rows),) + t1.shape)
+    for r, s in enumerate(rows):
+        c0, c1, c2, c3 = _CP_COEF[s]
+        cp[r] = (((c0 + c1 * t1) + c2 * t2) + c3 * t3) / GMW[s]
+        if icpc == 0 and s in (4, 5):
+            cp[r] = 1810.0 if s == 4 else 1004.0
     return cp

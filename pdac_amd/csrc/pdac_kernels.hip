@@ -1136,3 +1136,4 @@ __global__ void __launch_bounds__(kBlock, 3) k_tilde(Grid g, Consts K, Fields F,
 #include "pdac_ensemble.hip.inc"
 #include "pdac_io.hip.inc"
 #include "pdac_api.hip.inc"
+#include "pdac_setup.hip.inc"

@@ -146,6 +146,19 @@ MODULE pdac_gpu
       INTEGER(C_INT64_T), VALUE :: bytes
     END FUNCTION pdac_snapshot_fetch
 
+    INTEGER(C_INT) FUNCTION pdac_ib_forcing(nx, ny, nz, axis, ord2d, cx, cy, cz, topo2d, ext_cells, n_ext, &
+                                            cap, pts, nsl, count) BIND(C, name='pdac_ib_forcing')
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: nx, ny, nz, axis
+      INTEGER(C_INT32_T), INTENT(IN) :: ord2d(*)
+      REAL(C_DOUBLE), INTENT(IN) :: cx(*), cy(*), cz(*), topo2d(*)
+      INTEGER(C_INT64_T), INTENT(IN) :: ext_cells(*)
+      INTEGER(C_INT64_T), VALUE :: n_ext, cap
+      INTEGER(C_INT32_T), INTENT(OUT) :: pts(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: nsl(*)
+      INTEGER(C_INT64_T), INTENT(OUT) :: count
+    END FUNCTION pdac_ib_forcing
+
     INTEGER(C_INT) FUNCTION pdac_begin_step(ctx) BIND(C, name='pdac_begin_step')
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx

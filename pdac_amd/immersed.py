@@ -9,7 +9,9 @@ faces.  Its set-up, restated here in the reference's arithmetic:
   (interpolate_dem, topo.f:970-1029), cells with all but one or two faces
   under the ground become ``filled_cell_1/2`` and get external forcing
   points (ext_forcing3d, immb.f:1071-1261), the forcing points under the
-  ground (forcing3d, immb.f:913-1069), and ``immb_cell`` flags;
+  ground (forcing3d, immb.f:913-1069), and ``immb_cell`` flags; the
+  forcing points come from the library's pdac_ib_forcing (C++, host
+  threads over the mesh columns);
 * local_forcing (ghost.f:2888-3050): the cell of each forcing point and the
   neighbours its interpolation reads (index_q, index_qq);
 * fill_cells (ghost.f:3052-3261): per computed cell, which faces are above
@@ -24,7 +26,7 @@ Only 3D is restated (the 3D summit-vent case).
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List
 
 import numpy as np
@@ -32,34 +34,31 @@ import numpy as np
 from . import constants as C
 from .topography import TopoSetup, interp_2d
 
-# neighbour offsets of the forcing interpolation directions 1..8
-# (forcing3d, immb.f:940-955): W, SW, S, SE, E, NE, N, NW
-DI = [0, -1, -1, 0, 1, 1, 1, 0, -1]
-DJ = [0, 0, -1, -1, -1, 0, 1, 1, 1]
-
-
 @dataclass
 class ForcingPoints:
-    """Forcing points of one velocity component, in the reference's order;
-    i, j, k 1-based."""
-    i: List[int] = field(default_factory=list)
-    j: List[int] = field(default_factory=list)
-    k: List[int] = field(default_factory=list)
-    int_: List[int] = field(default_factory=list)
-    di: List[int] = field(default_factory=list)
-    dj: List[int] = field(default_factory=list)
-    dk: List[int] = field(default_factory=list)
-    nsx: List[float] = field(default_factory=list)
-    nsy: List[float] = field(default_factory=list)
-    nsz: List[float] = field(default_factory=list)
-
-    def add(self, i, j, k, it, di, dj, dk, x, y, z):
-        self.i.append(i); self.j.append(j); self.k.append(k); self.int_.append(it)
-        self.di.append(di); self.dj.append(dj); self.dk.append(dk)
-        self.nsx.append(x); self.nsy.append(y); self.nsz.append(z)
+    """Forcing points of one velocity component, in the reference's order
+    (TYPE forcing_point, immb.f:33-45): int32 arrays i, j, k (1-based),
+    int_ (interpolation type), di, dj, dk (direction), float64 arrays nsx,
+    nsy, nsz (the no-slip point)."""
+    i: np.ndarray
+    j: np.ndarray
+    k: np.ndarray
+    int_: np.ndarray
+    di: np.ndarray
+    dj: np.ndarray
+    dk: np.ndarray
+    nsx: np.ndarray
+    nsy: np.ndarray
+    nsz: np.ndarray
 
     def __len__(self):
         return len(self.i)
+
+
+def _cells(fp: ForcingPoints, nx, ny):
+    """0-based cell of each point."""
+    i, j, k = (np.asarray(v, dtype=np.int64) for v in (fp.i, fp.j, fp.k))
+    return (i - 1) + (j - 1) * nx + (k - 1) * nx * ny
 
 
 @dataclass
@@ -108,93 +107,36 @@ class _DemState:
         return topo2d
 
 
-def _gamma(topo2d, cz, i, j, k, n):
-    num = cz[k - 1] - topo2d[i - 1, j - 1]
-    den = topo2d[i + DI[n] - 1, j + DJ[n] - 1] - topo2d[i - 1, j - 1]
-    return num / den
+def _forcing_points(o, grid, topo2d, ext_cells, axis, nz, cap) -> ForcingPoints:
+    """The forcing points of one velocity component from the library's
+    pdac_ib_forcing: ext_forcing3d for each cell of ext_cells, then forcing3d
+    over the interior columns (immb.f:913-1261), on host threads."""
+    import ctypes as ct
 
-
-def _forcing3d(o, cx, cy, cz, topo2d, fp: ForcingPoints, nx, ny):
-    """forcing3d (immb.f:913-1069): loop i, j, k; the interpolation
-    neighbour is the one whose gamma is nearest 0.5 (mingam, first on
-    ties)."""
-    cx, cy, cz = (np.asarray(a, dtype=np.float64) for a in (cx, cy, cz))
-    recs = []
-    kmax = int(o[1:-1, 1:-1].max())
-    for k in range(2, kmax):
-        ii, jj = np.nonzero(o[1:-1, 1:-1] - 1 >= k)
-        ii, jj = ii + 2, jj + 2                      # 1-based
-        if ii.size == 0:
-            continue
-        g = np.full((9, ii.size), -1.0)
-        for n in range(1, 9):
-            on = o[ii + DI[n] - 1, jj + DJ[n] - 1]
-            sel = on < k
-            num = cz[k - 1] - topo2d[ii - 1, jj - 1]
-            den = topo2d[ii + DI[n] - 1, jj + DJ[n] - 1] - topo2d[ii - 1, jj - 1]
-            with np.errstate(divide="ignore", invalid="ignore"):
-                g[n] = np.where(sel, num / den, -1.0)
-        has = np.max(g[1:], axis=0) > -1
-        d = np.abs(0.5 - g[1:])
-        gint = np.argmin(d, axis=0) + 1
-        if np.any(has & (np.min(d, axis=0) >= 1.0)):
-            raise ValueError("forcing3d: no interpolation direction (mingam)")
-        for p in np.flatnonzero(has):
-            recs.append((int(ii[p]), int(jj[p]), k, int(gint[p]), float(g[gint[p], p])))
-    for i in range(2, nx):
-        for j in range(2, ny):
-            recs.append((i, j, int(o[i - 1, j - 1]), 0, 0.0))
-    recs.sort(key=lambda r: (r[0], r[1], r[2]))
-    for i, j, k, gi, gg in recs:
-        if gi == 0:
-            fp.add(i, j, k, 0, 0, 0, 1, cx[i - 1], cy[j - 1], topo2d[i - 1, j - 1])
-        else:
-            fp.add(i, j, k, gi, DI[gi], DJ[gi], 0,
-                   gg * cx[i - 1] + (1.0 - gg) * cx[i + DI[gi] - 1],
-                   gg * cy[j - 1] + (1.0 - gg) * cy[j + DJ[gi] - 1],
-                   cz[k - 1])
-
-
-def _ext_forcing3d(o, i, j, k, cx, cy, cz, topo2d, fp: ForcingPoints, axis):
-    """ext_forcing3d (immb.f:1071-1261) for a cell whose component lies
-    above the ground."""
-    g = [None] + [-1.0] * 8
-    g_ext = [None, 0.0, 0.0, 0.0, 0.0]
-    maxg = 1.0
-    g_min = None
-    gint = fpint = None
-    if k > o[i - 1, j - 1]:
-        for n in range(1, 9):
-            if o[i + DI[n] - 1, j + DJ[n] - 1] >= k:
-                g[n] = _gamma(topo2d, cz, i, j, k, n)
-        if axis != 1:
-            g_ext[1] = g[1] * g[5]
-        g_ext[2] = g[2] * g[6]
-        if axis != 2:
-            g_ext[3] = g[3] * g[7]
-        g_ext[4] = g[4] * g[8]
-        for l in range(1, 5):
-            if g_ext[l] < 0 and -g_ext[l] < maxg:
-                g_min = l
-                maxg = -g_ext[l]
-        if min(g_ext[1:]) < 0 and g_min is not None:
-            a, b = g_min, g_min + 4
-            if g[a] < 0:
-                gint, fpint = b, 20 + a
-            else:
-                gint, fpint = a, 24 + a
-    if k == o[i - 1, j - 1] + 1:
-        maxg_z = (cz[k - 1] - topo2d[i - 1, j - 1]) / (cz[k] - topo2d[i - 1, j - 1])
-    else:
-        maxg_z = 1.0
-    if maxg < 1 and maxg < maxg_z:
-        gg = g[gint]
-        fp.add(i, j, k, fpint, DI[fpint - 20], DJ[fpint - 20], 0,
-               (1.0 - gg) * cx[i - 1] + gg * cx[i + DI[gint] - 1],
-               (1.0 - gg) * cy[j - 1] + gg * cy[j + DJ[gint] - 1],
-               cz[k - 1])
-    else:
-        fp.add(i, j, k, 20, 0, 0, 1, cx[i - 1], cy[j - 1], topo2d[i - 1, j - 1])
+    from .gpu import lib
+    L = lib()
+    f = L.pdac_ib_forcing
+    i32, i64, f64 = ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int64), ct.POINTER(ct.c_double)
+    f.argtypes = [ct.c_int32] * 4 + [i32, f64, f64, f64, f64, i64, ct.c_int64, ct.c_int64, i32, f64, i64]
+    f.restype = ct.c_int
+    nx, ny = o.shape
+    o32 = np.ascontiguousarray(o, dtype=np.int32)
+    cx, cy, cz = (np.ascontiguousarray(g, dtype=np.float64) for g in grid)
+    t = np.ascontiguousarray(topo2d, dtype=np.float64)
+    ext = np.ascontiguousarray(ext_cells, dtype=np.int64)
+    pts = np.empty((max(cap, 1), 7), dtype=np.int32)
+    nsl = np.empty((max(cap, 1), 3), dtype=np.float64)
+    count = ct.c_int64(0)
+    p = lambda a, ty: a.ctypes.data_as(ty)  # noqa: E731
+    rc = f(nx, ny, nz, axis, p(o32, i32), p(cx, f64), p(cy, f64), p(cz, f64), p(t, f64), p(ext, i64),
+           ext.size, cap, p(pts, i32), p(nsl, f64), ct.byref(count))
+    if rc != 0:
+        if count.value > cap:
+            raise ValueError(f"set_forcing: {count.value} forcing points along axis {axis}, "
+                             f"{cap} counted (the reference would fail)")
+        raise ValueError(L.pdac_last_error().decode())
+    n = count.value
+    return ForcingPoints(*(pts[:n, c].copy() for c in range(7)), *(nsl[:n, c].copy() for c in range(3)))
 
 
 def immersed_setup(case, ts: TopoSetup, flags_in: np.ndarray) -> ImmersedSetup:
@@ -220,54 +162,46 @@ def immersed_setup(case, ts: TopoSetup, flags_in: np.ndarray) -> ImmersedSetup:
     zt = m.zb[1:-1, None, None]
     zbm = m.zb[:-2, None, None]
     tci = tc[1:-1, 1:-1].T[None]
-    bds = ((zc > tx[1:-1, 1:-1].T[None]) * 1 + (zc > tx[:-2, 1:-1].T[None]) * 2 +
-           (zt > tci) * 4 + (zbm >= tci) * 8 +
-           (zc > ty[1:-1, 1:-1].T[None]) * 16 + (zc > ty[1:-1, :-2].T[None]) * 32)
+    full = np.full((nz, ny, nx), 255, dtype=np.uint8)   # 255: the mesh's outer cells
+    bds = full[1:-1, 1:-1, 1:-1]
+    bds[...] = zc > tx[1:-1, 1:-1].T[None]
+    bds |= (zc > tx[:-2, 1:-1].T[None]).view(np.uint8) << 1
+    bds |= (zt > tci).view(np.uint8) << 2
+    bds |= (zbm >= tci).view(np.uint8) << 3
+    bds |= (zc > ty[1:-1, 1:-1].T[None]).view(np.uint8) << 4
+    bds |= (zc > ty[1:-1, :-2].T[None]).view(np.uint8) << 5
     if np.any(bds == 8):
         raise ValueError("set_forcing: inconsistent immersed boundary (bds = 8)")
-    full = np.zeros((nz, ny, nx), dtype=np.int64)
-    full[1:-1, 1:-1, 1:-1] = bds
-    full[0, :, :] = -1
-    full[-1, :, :] = -1
-    full[:, 0, :] = -1
-    full[:, -1, :] = -1
-    full[:, :, 0] = -1
-    full[:, :, -1] = -1
     b = full.reshape(-1)
-    own_x = np.isin(b, (1, 17, 33, 3))
-    west_x = np.isin(b, (2, 18, 34, 3))
-    own_y = np.isin(b, (16, 18, 17, 48))
-    south_y = np.isin(b, (32, 34, 33, 48))
-    ext[0] |= own_x
-    ext[0][np.flatnonzero(west_x) - 1] = True
-    ext[1] |= own_y
-    ext[1][np.flatnonzero(south_y) - nx] = True
+
+    def of(*vals):   # cells whose open-face bits are one of vals
+        lut = np.zeros(256, dtype=bool)
+        lut[list(vals)] = True
+        return lut[b]
+
+    ext[0] |= of(1, 17, 33, 3)
+    ext[0][np.flatnonzero(of(2, 18, 34, 3)) - 1] = True
+    ext[1] |= of(16, 18, 17, 48)
+    ext[1][np.flatnonzero(of(32, 34, 33, 48)) - nx] = True
     ext[2] |= b == 4
-    f1 = np.isin(b, (1, 2, 4, 16, 32)) & ~np.isin(fl, keep)
-    f2 = np.isin(b, (17, 18, 34, 33, 48, 3)) & ~np.isin(fl, keep)
+    free = ~np.isin(fl, keep)
+    f1 = of(1, 2, 4, 16, 32) & free
+    f2 = of(17, 18, 34, 33, 48, 3) & free
     fl[f1] = C.FILLED_CELL_1
     fl[f2] = C.FILLED_CELL_2
     counts = [int(force[a].sum() + ext[a].sum()) for a in range(3)]
     fps = []
     for a in range(3):
         topo2d = st.call(*grids[a], force[a])
-        cx, cy, cz = grids[a]
-        fp = ForcingPoints()
-        for c in np.flatnonzero(ext[a]):
-            k = c // (nx * ny) + 1
-            j = (c // nx) % ny + 1
-            i = c % nx + 1
-            _ext_forcing3d(st.ord2d, i, j, k, cx, cy, cz, topo2d, fp, a + 1)
-        _forcing3d(st.ord2d, cx, cy, cz, topo2d, fp, nx, ny)
+        fp = _forcing_points(st.ord2d, grids[a], topo2d, np.flatnonzero(ext[a]), a + 1, nz, counts[a])
         if len(fp) != counts[a]:
             raise ValueError(f"set_forcing: {len(fp)} forcing points along axis {a + 1}, "
                              f"{counts[a]} counted (the reference would fail)")
-        for p in range(len(fp)):
-            i, j, k = fp.i[p], fp.j[p], fp.k[p]
-            c = (i - 1) + (j - 1) * nx + (k - 1) * nx * ny
-            if i > 1 and j > 1 and k > 1 and fl[c] not in (C.INLET_CELL, C.VENT_CELL,
-                                                             C.FILLED_CELL_1, C.FILLED_CELL_2):
-                fl[c] = C.IMMB_CELL
+        # the cells of the points become immb_cell unless inlet, vent or filled
+        c = _cells(fp, nx, ny)
+        c = c[(fp.i > 1) & (fp.j > 1) & (fp.k > 1)]
+        c = c[~np.isin(fl[c], (C.INLET_CELL, C.VENT_CELL, C.FILLED_CELL_1, C.FILLED_CELL_2))]
+        fl[c] = C.IMMB_CELL
         fps.append(fp)
     # write_profile (topo.f:1225-1255): the cells under a specified-flow
     # block of type 5 are no-slip walls again, whatever set_forcing made
@@ -282,18 +216,14 @@ def immersed_setup(case, ts: TopoSetup, flags_in: np.ndarray) -> ImmersedSetup:
     iq, iqq = [], []
     for a in range(3):
         fp = fps[a]
-        q = np.empty(len(fp), dtype=np.int64)
-        qq = np.full(len(fp), -1, dtype=np.int64)
-        for p in range(len(fp)):
-            i, j, k = fp.i[p], fp.j[p], fp.k[p]
-            c = (i - 1) + (j - 1) * nx + (k - 1) * nx * ny
-            num[a, c] = p + 1
-            q[p] = c + fp.di[p] + fp.dj[p] * nx + fp.dk[p] * nx * ny
-            it = fp.int_[p]
-            if it % 2 != 0 or it == 0:
-                qq[p] = c + 2 * fp.di[p] + 2 * fp.dj[p] * nx + 2 * fp.dk[p] * nx * ny
-        iq.append(q)
-        iqq.append(qq)
+        c = _cells(fp, nx, ny)
+        # num: the last point of a cell (points are numbered in order)
+        np.maximum.at(num[a], c, np.arange(1, len(fp) + 1, dtype=np.int64))
+        di, dj, dk = (np.asarray(v, dtype=np.int64) for v in (fp.di, fp.dj, fp.dk))
+        step = di + dj * nx + dk * nx * ny
+        it = np.asarray(fp.int_, dtype=np.int64)
+        iq.append(c + step)
+        iqq.append(np.where((it % 2 != 0) | (it == 0), c + 2 * step, -1))
     # fill_cells (ghost.f:3052-3261), in the reference's cell order: a cell
     # left with no open face becomes filled_cell_1 and later cells see it
     # so; within one level that cannot change a neighbour's count (the

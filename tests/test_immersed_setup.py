@@ -15,8 +15,26 @@ from tests.golden.make_golden import load, load_extras
 IB_CASES = ["cone_3d_ib", "cone_3d_ib_steep"]
 
 
+FP_FIELDS = ("i", "j", "k", "int_", "di", "dj", "dk", "nsx", "nsy", "nsz")
+
+
+def _restated(o, grid, topo2d, ext_cells, axis, nz, cap):
+    """The oracle's restatement in place of the library's pdac_ib_forcing."""
+    import pdac_amd.immersed as im
+    from oracle.ib_forcing import forcing_points
+    nx, ny = o.shape
+    p = forcing_points(o, grid, topo2d, ext_cells, axis, nx, ny)
+    return im.ForcingPoints(*(np.asarray(getattr(p, f)) for f in FP_FIELDS))
+
+
+@pytest.mark.parametrize("source", ["library", "oracle"])
 @pytest.mark.parametrize("name", IB_CASES)
-def test_forcing_points_match_the_reference(name):
+def test_forcing_points_match_the_reference(name, source, monkeypatch):
+    """The forcing points of the library (pdac_ib_forcing, host threads) and
+    of the oracle's restatement against the reference's own dump."""
+    if source == "oracle":
+        import pdac_amd.immersed as im
+        monkeypatch.setattr(im, "_forcing_points", _restated)
     case, _, _ = load(name)
     ref = load_extras(name)
     ib = case.immersed()
@@ -33,6 +51,31 @@ def test_forcing_points_match_the_reference(name):
     fl = ib.flags
     for f in (C.IMMB_CELL, C.FILLED_CELL_1, C.INLET_CELL):
         assert np.any(fl == f), f
+
+
+def test_library_forcing_points_equal_the_restatement(monkeypatch):
+    """pdac_ib_forcing against the oracle's restatement, bit for bit, on
+    rough cones larger than the fixtures (every column split over threads,
+    thousands of points of each type)."""
+    import pdac_amd.immersed as im
+    from pdac_amd.cases import summit_cone_3d
+    real, seen = im._forcing_points, []
+
+    def both(o, grid, topo2d, ext_cells, axis, nz, cap):
+        got = real(o, grid, topo2d, ext_cells, axis, nz, cap)
+        want = _restated(o, grid, topo2d, ext_cells, axis, nz, cap)
+        for f in FP_FIELDS:
+            np.testing.assert_array_equal(getattr(got, f), getattr(want, f), err_msg=f"axis {axis} {f}")
+        seen.append(np.unique(got.int_))
+        return got
+
+    monkeypatch.setattr(im, "_forcing_points", both)
+    for seed, rough in ((7, 12.0), (8, 25.0)):
+        summit_cone_3d(nx=64, ny=56, nz=40, d=10.0, height=250.0, sigma=150.0, rough=rough,
+                       vent_half=20.0, nsteps=1, seed=seed).immersed()
+    assert len(seen) == 6
+    types = set(np.concatenate(seen).tolist())
+    assert 0 in types and 20 in types and any(1 <= t <= 8 for t in types) and any(t > 20 for t in types)
 
 
 @pytest.mark.parametrize("bit,what", [(1, "boundary forcing"), (2, "tilde forcing"),
