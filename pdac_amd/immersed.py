@@ -280,8 +280,7 @@ def immersed_setup(case, ts: TopoSetup, flags_in: np.ndarray) -> ImmersedSetup:
         vfv[k - 1] = np.where(comp, vv, 0.0)
         lay[comp & (vv == 0.0)] = C.FILLED_CELL_1
     # face_fractions (ghost.f:3263-3321)
-    bmask = np.ones((6, n), dtype=np.int64)   # e, w, n, s, t, b
-    ivf = np.ones(n)
+    bmask = np.empty((6, n), dtype=np.int32)   # e, w, n, s, t, b
     comp = (fl & 1) == 1
     stop = np.flatnonzero(comp & (num[0] != 0) & (num[2] != 0))
     ib_last = int(stop[0]) if stop.size else n - 1
@@ -289,9 +288,9 @@ def immersed_setup(case, ts: TopoSetup, flags_in: np.ndarray) -> ImmersedSetup:
     upto[:ib_last if stop.size else n] = True
     sel = comp & upto
     for row, bit in ((0, 1), (1, 2), (4, 4), (5, 8), (2, 16), (3, 32)):
-        bmask[row, sel] = (bd[sel] & bit != 0).astype(np.int64)
-    with np.errstate(divide="ignore"):
-        ivf[sel] = np.where(vf[sel] > 0.0, 1.0 / np.where(vf[sel] > 0.0, vf[sel], 1.0), 0.0)
+        np.logical_or(~sel, (bd & bit) != 0, out=bmask[row], casting="unsafe")
+    pos = vf > 0.0
+    ivf = np.where(sel, np.where(pos, 1.0 / np.where(pos, vf, 1.0), 0.0), 1.0)
     if stop.size:
         bmask[[0, 1, 4, 5], ib_last] = 0
     return ImmersedSetup(fl, fps, [list(g) for g in grids], num, iq, iqq, bd, vf, bmask, ivf, ib_last)
