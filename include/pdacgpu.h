@@ -205,6 +205,16 @@ int pdac_ib_forcing(int32_t nx, int32_t ny, int32_t nz, int32_t axis, const int3
                     const double *cy, const double *cz, const double *topo2d, const int64_t *ext_cells,
                     int64_t n_ext, int64_t cap, int32_t *pts, double *nsl, int64_t *count);
 
+/* Immersed-boundary set-up on the host (no device call): fill_cells
+ * (ghost.f:3052-3261) in the reference's serial cell order.  For each
+ * computed interior cell of flags (ntot values, updated in place), the open
+ * faces bd (bits e w t b n s = 1 2 4 8 16 32) and open volume fraction vf;
+ * a cell with no open face becomes filled_cell_1 (512).  tx, ty, tc: the
+ * ground heights of the x, y and cell-centred grids at (i-1)*ny + (j-1); z,
+ * zb: the cell centres and upper faces along z (nz values each). */
+int pdac_ib_fill_cells(int32_t nx, int32_t ny, int32_t nz, const double *tx, const double *ty, const double *tc,
+                       const double *z, const double *zb, int32_t *flags, int32_t *bd, double *vf);
+
 /* Host <-> device copies of one field; n must equal ntot*ncomp(field). */
 int pdac_put(pdac_ctx *ctx, int field, const double *host, int64_t n);
 int pdac_get(pdac_ctx *ctx, int field, double *host, int64_t n);

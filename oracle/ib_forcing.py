@@ -1,6 +1,7 @@
 """Oracle (test infrastructure only): the forcing points of one velocity
-component restated in numpy and Python loops, the checker of the library's
-pdac_ib_forcing.  Only tests import this module.
+component and fill_cells restated in numpy and Python loops, the checkers of
+the library's pdac_ib_forcing and pdac_ib_fill_cells.  Only tests import
+this module.
 
 forcing3d (immb.f:913-1069): for each interior column and each level below
 its ground level, the interpolation direction whose gamma is nearest 0.5
@@ -146,3 +147,70 @@ def _ext_forcing3d(o, i, j, k, cx, cy, cz, topo2d, fp: "Points", axis):
                cz[k - 1])
     else:
         fp.add(i, j, k, 20, 0, 0, 1, cx[i - 1], cy[j - 1], topo2d[i - 1, j - 1])
+
+
+FILLED_CELL_1, FILLED_CELL_2 = 512, 1024
+
+
+def fill_cells(fl, tx, ty, tc, z, zb, nx, ny, nz):
+    """fill_cells (ghost.f:3052-3261) level by level in numpy: the flags with
+    closed cells made filled_cell_1, the open-face bits bd and the open
+    volume fraction vf (the checker of the library's pdac_ib_fill_cells)."""
+    fl = np.array(fl, copy=True)
+    n = nx * ny * nz
+    # fill_cells (ghost.f:3052-3261), in the reference's cell order: a cell
+    # left with no open face becomes filled_cell_1 and later cells see it
+    # so; within one level that cannot change a neighbour's count (the
+    # shared face is closed for both), across levels it can, so the levels
+    # go in order
+    bd = np.zeros(n, dtype=np.int64)
+    vf = np.zeros(n)
+    flv = fl.reshape(nz, ny, nx)
+    bdv = bd.reshape(nz, ny, nx)
+    vfv = vf.reshape(nz, ny, nx)
+    for k in range(2, nz):
+        lay = flv[k - 1]
+        comp = (lay & 1) == 1
+        comp[0, :] = comp[-1, :] = comp[:, 0] = comp[:, -1] = False
+        zk, zbk, zbm1 = z[k - 1], zb[k - 1], zb[k - 2]
+        bb = np.zeros((ny, nx), dtype=np.int64)
+        vv = np.zeros((ny, nx))
+
+        def side(open_, nbf, bit):
+            nonlocal bb, vv
+            f1_ = open_ & (nbf == FILLED_CELL_1)
+            f2_ = open_ & (nbf == FILLED_CELL_2)
+            other = open_ & ~f1_ & ~f2_
+            vv = vv + np.where(f1_, 2.0, 0.0) + np.where(f2_, 1.0, 0.0) + np.where(other, 1.0, 0.0)
+            bb = bb + np.where(other, bit, 0)
+
+        sl = np.s_[1:-1, 1:-1]
+        op = np.zeros((ny, nx), dtype=bool)
+        nb = np.zeros((ny, nx), dtype=np.int64)
+        # east, west
+        op[sl] = zk > tx[1:-1, 1:-1].T
+        nb[sl] = lay[1:-1, 2:]
+        side(op.copy(), nb.copy(), 1)
+        op[sl] = zk > tx[:-2, 1:-1].T
+        nb[sl] = lay[1:-1, :-2]
+        side(op.copy(), nb.copy(), 2)
+        # top
+        op[sl] = zbk > tc[1:-1, 1:-1].T
+        vv = vv + np.where(op, 1.0, 0.0)
+        bb = bb + np.where(op, 4, 0)
+        # bottom
+        op[sl] = zbm1 >= tc[1:-1, 1:-1].T
+        nb[sl] = flv[k - 2][1:-1, 1:-1]
+        side(op.copy(), nb.copy(), 8)
+        # north, south
+        op[sl] = zk > ty[1:-1, 1:-1].T
+        nb[sl] = lay[2:, 1:-1]
+        side(op.copy(), nb.copy(), 16)
+        op[sl] = zk > ty[1:-1, :-2].T
+        nb[sl] = lay[:-2, 1:-1]
+        side(op.copy(), nb.copy(), 32)
+        vv = vv / 6
+        bdv[k - 1] = np.where(comp, bb, 0)
+        vfv[k - 1] = np.where(comp, vv, 0.0)
+        lay[comp & (vv == 0.0)] = FILLED_CELL_1
+    return fl, bd, vf

@@ -159,6 +159,16 @@ MODULE pdac_gpu
       INTEGER(C_INT64_T), INTENT(OUT) :: count
     END FUNCTION pdac_ib_forcing
 
+    INTEGER(C_INT) FUNCTION pdac_ib_fill_cells(nx, ny, nz, tx, ty, tc, z, zb, flags, bd, vf) &
+        BIND(C, name='pdac_ib_fill_cells')
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: nx, ny, nz
+      REAL(C_DOUBLE), INTENT(IN) :: tx(*), ty(*), tc(*), z(*), zb(*)
+      INTEGER(C_INT32_T), INTENT(INOUT) :: flags(*)
+      INTEGER(C_INT32_T), INTENT(OUT) :: bd(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: vf(*)
+    END FUNCTION pdac_ib_fill_cells
+
     INTEGER(C_INT) FUNCTION pdac_begin_step(ctx) BIND(C, name='pdac_begin_step')
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx

@@ -27,6 +27,7 @@ EXPORTS = [
     "pdac_ens_create", "pdac_ens_destroy", "pdac_ens_tune", "pdac_ens_set_tiles", "pdac_ens_step",
     "pdac_explicit_step_noalloc", "pdac_set_ib", "pdac_profile_checks", "pdac_sgsg",
     "pdac_snapshot_size", "pdac_snapshot_async", "pdac_snapshot_fetch", "pdac_ib_forcing",
+    "pdac_ib_fill_cells",
 ]
 
 

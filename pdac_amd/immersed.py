@@ -139,6 +139,29 @@ def _forcing_points(o, grid, topo2d, ext_cells, axis, nz, cap) -> ForcingPoints:
     return ForcingPoints(*(pts[:n, c].copy() for c in range(7)), *(nsl[:n, c].copy() for c in range(3)))
 
 
+def _fill_cells(fl, tx, ty, tc, z, zb, nx, ny, nz):
+    """fill_cells from the library's pdac_ib_fill_cells (the reference's
+    serial cell loop): flags with the closed cells made filled_cell_1, the
+    open-face bits bd and the open volume fraction vf."""
+    import ctypes as ct
+
+    from .gpu import lib
+    L = lib()
+    f = L.pdac_ib_fill_cells
+    i32, f64 = ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double)
+    f.argtypes = [ct.c_int32] * 3 + [f64] * 5 + [i32, i32, f64]
+    f.restype = ct.c_int
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (tx, ty, tc, z, zb)]
+    out_fl = np.ascontiguousarray(fl, dtype=np.int32).copy()
+    bd = np.empty(out_fl.size, dtype=np.int32)
+    vf = np.empty(out_fl.size, dtype=np.float64)
+    rc = f(nx, ny, nz, *(a.ctypes.data_as(f64) for a in arrs), out_fl.ctypes.data_as(i32),
+           bd.ctypes.data_as(i32), vf.ctypes.data_as(f64))
+    if rc != 0:
+        raise ValueError(L.pdac_last_error().decode())
+    return out_fl.astype(fl.dtype), bd.astype(np.int64), vf
+
+
 def immersed_setup(case, ts: TopoSetup, flags_in: np.ndarray) -> ImmersedSetup:
     """set_forcing, local_forcing, fill_cells and face_fractions for a 3D
     case; ``flags_in`` are the flags after export_topography."""
@@ -224,61 +247,8 @@ def immersed_setup(case, ts: TopoSetup, flags_in: np.ndarray) -> ImmersedSetup:
         it = np.asarray(fp.int_, dtype=np.int64)
         iq.append(c + step)
         iqq.append(np.where((it % 2 != 0) | (it == 0), c + 2 * step, -1))
-    # fill_cells (ghost.f:3052-3261), in the reference's cell order: a cell
-    # left with no open face becomes filled_cell_1 and later cells see it
-    # so; within one level that cannot change a neighbour's count (the
-    # shared face is closed for both), across levels it can, so the levels
-    # go in order
-    bd = np.zeros(n, dtype=np.int64)
-    vf = np.zeros(n)
-    flv = fl.reshape(nz, ny, nx)
-    bdv = bd.reshape(nz, ny, nx)
-    vfv = vf.reshape(nz, ny, nx)
-    for k in range(2, nz):
-        lay = flv[k - 1]
-        comp = (lay & 1) == 1
-        comp[0, :] = comp[-1, :] = comp[:, 0] = comp[:, -1] = False
-        zk, zbk, zbm1 = m.z[k - 1], m.zb[k - 1], m.zb[k - 2]
-        bb = np.zeros((ny, nx), dtype=np.int64)
-        vv = np.zeros((ny, nx))
-
-        def side(open_, nbf, bit):
-            nonlocal bb, vv
-            f1_ = open_ & (nbf == C.FILLED_CELL_1)
-            f2_ = open_ & (nbf == C.FILLED_CELL_2)
-            other = open_ & ~f1_ & ~f2_
-            vv = vv + np.where(f1_, 2.0, 0.0) + np.where(f2_, 1.0, 0.0) + np.where(other, 1.0, 0.0)
-            bb = bb + np.where(other, bit, 0)
-
-        sl = np.s_[1:-1, 1:-1]
-        op = np.zeros((ny, nx), dtype=bool)
-        nb = np.zeros((ny, nx), dtype=np.int64)
-        # east, west
-        op[sl] = zk > tx[1:-1, 1:-1].T
-        nb[sl] = lay[1:-1, 2:]
-        side(op.copy(), nb.copy(), 1)
-        op[sl] = zk > tx[:-2, 1:-1].T
-        nb[sl] = lay[1:-1, :-2]
-        side(op.copy(), nb.copy(), 2)
-        # top
-        op[sl] = zbk > tc[1:-1, 1:-1].T
-        vv = vv + np.where(op, 1.0, 0.0)
-        bb = bb + np.where(op, 4, 0)
-        # bottom
-        op[sl] = zbm1 >= tc[1:-1, 1:-1].T
-        nb[sl] = flv[k - 2][1:-1, 1:-1]
-        side(op.copy(), nb.copy(), 8)
-        # north, south
-        op[sl] = zk > ty[1:-1, 1:-1].T
-        nb[sl] = lay[2:, 1:-1]
-        side(op.copy(), nb.copy(), 16)
-        op[sl] = zk > ty[1:-1, :-2].T
-        nb[sl] = lay[:-2, 1:-1]
-        side(op.copy(), nb.copy(), 32)
-        vv = vv / 6
-        bdv[k - 1] = np.where(comp, bb, 0)
-        vfv[k - 1] = np.where(comp, vv, 0.0)
-        lay[comp & (vv == 0.0)] = C.FILLED_CELL_1
+    # fill_cells (ghost.f:3052-3261), in the reference's cell order
+    fl, bd, vf = _fill_cells(fl, tx, ty, tc, m.z, m.zb, nx, ny, nz)
     # face_fractions (ghost.f:3263-3321)
     bmask = np.empty((6, n), dtype=np.int32)   # e, w, n, s, t, b
     comp = (fl & 1) == 1

@@ -34,7 +34,9 @@ def test_forcing_points_match_the_reference(name, source, monkeypatch):
     of the oracle's restatement against the reference's own dump."""
     if source == "oracle":
         import pdac_amd.immersed as im
+        from oracle.ib_forcing import fill_cells
         monkeypatch.setattr(im, "_forcing_points", _restated)
+        monkeypatch.setattr(im, "_fill_cells", fill_cells)
     case, _, _ = load(name)
     ref = load_extras(name)
     ib = case.immersed()
@@ -54,12 +56,23 @@ def test_forcing_points_match_the_reference(name, source, monkeypatch):
 
 
 def test_library_forcing_points_equal_the_restatement(monkeypatch):
-    """pdac_ib_forcing against the oracle's restatement, bit for bit, on
-    rough cones larger than the fixtures (every column split over threads,
-    thousands of points of each type)."""
+    """pdac_ib_forcing and pdac_ib_fill_cells against the oracle's
+    restatements, bit for bit, on rough cones larger than the fixtures
+    (every column split over threads, thousands of points of each type,
+    cells closed by fill_cells)."""
     import pdac_amd.immersed as im
+    from oracle.ib_forcing import fill_cells
     from pdac_amd.cases import summit_cone_3d
     real, seen = im._forcing_points, []
+    real_fill, filled = im._fill_cells, []
+
+    def both_fill(fl, tx, ty, tc, z, zb, nx, ny, nz):
+        got = real_fill(fl, tx, ty, tc, z, zb, nx, ny, nz)
+        want = fill_cells(fl, tx, ty, tc, z, zb, nx, ny, nz)
+        for a, b, what in zip(got, want, ("flags", "bd", "vf")):
+            np.testing.assert_array_equal(a, b, err_msg=what)
+        filled.append(int(np.count_nonzero(got[0] != fl)))
+        return got
 
     def both(o, grid, topo2d, ext_cells, axis, nz, cap):
         got = real(o, grid, topo2d, ext_cells, axis, nz, cap)
@@ -70,10 +83,11 @@ def test_library_forcing_points_equal_the_restatement(monkeypatch):
         return got
 
     monkeypatch.setattr(im, "_forcing_points", both)
+    monkeypatch.setattr(im, "_fill_cells", both_fill)
     for seed, rough in ((7, 12.0), (8, 25.0)):
         summit_cone_3d(nx=64, ny=56, nz=40, d=10.0, height=250.0, sigma=150.0, rough=rough,
                        vent_half=20.0, nsteps=1, seed=seed).immersed()
-    assert len(seen) == 6
+    assert len(seen) == 6 and len(filled) == 2 and min(filled) > 0
     types = set(np.concatenate(seen).tolist())
     assert 0 in types and 20 in types and any(1 <= t <= 8 for t in types) and any(t > 20 for t in types)
 
