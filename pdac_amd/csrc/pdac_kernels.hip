@@ -925,8 +925,13 @@ __global__ void __launch_bounds__(kBlock) k_drag(Grid g, Consts K, Fields F) {
     const double dugs = ((ug[c] - us[os + c]) + (ug[q.imjk] - us[os + q.imjk])) * 0.5;
     const double dwgs = ((wg[c] - ws[os + c]) + (wg[q.ijkm] - ws[os + q.ijkm])) * 0.5;
     const double dvgs = (D == 3) ? ((vg[c] - vs[os + c]) + (vg[q.ijmk] - vs[os + q.ijmk])) * 0.5 : 0.0;
-    F.kpgv[os + c] = K.drag_model == 1 ? kdrags(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c])
-                                       : sdrag(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rlk[os + c], mug[c]);
+    // with no particles of the class here both laws give the density times
+    // nonnegative finite factors: a zero of the density's sign, without the
+    // powers and roots (most cells of a plume hold no particles)
+    const double rl = rlk[os + c];
+    F.kpgv[os + c] = rl == 0.0 ? rl * 0.0
+                   : K.drag_model == 1 ? kdrags(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rl, mug[c])
+                                       : sdrag(K, is, dugs, dvgs, dwgs, ep[c], rgp[c], rl, mug[c]);
   }
 }
 
