@@ -48,10 +48,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per working launch of the sweep kernel (k_sweep, one colour
 # phase) on the default workload (FP32-flux form): rocprofv3 FETCH_SIZE and
 # WRITE_SIZE over its dispatches, each scaled by what the counter reads for a
-# copy of known size (--copy-calibration): 45.41 MB read + 6.54 MB written
+# copy of known size (--copy-calibration): 45.47 MB read + 6.55 MB written
 # over the 50,772 working dispatches of the default event region (steps
 # 260-360); None until measured
-SWEEP_TRAFFIC_PER_LAUNCH = 51.95e6
+SWEEP_TRAFFIC_PER_LAUNCH = 52.02e6
 # order of enum pdac_kernel (include/pdacgpu.h)
 KERNELS = ["boundary", "fieldn", "momflux", "tilde", "predict", "gasflux", "snapshot",
            "sweep", "ygas_flux", "ygas_update", "htilde_flux", "htilde_update", "ftem", "fill",
